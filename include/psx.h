@@ -1,0 +1,173 @@
+/*
+ * psx.h - C ABI of the MI355X-native acceleration-search core (libpsx.so).
+ *
+ * This is the drop-in boundary below peasoup's C++ transform classes.  Each
+ * entry point replaces one function of the reference's kernel layer
+ * (include/kernels/kernels.h) or one transform method, cited per function as
+ * file:line of the reference source tree.  Conventions:
+ *   - every function returns an int status (PSX_OK == 0); on failure
+ *     psx_last_error_string() describes the error of the calling thread;
+ *   - device pointers are plain pointers from psx_malloc (or any HIP
+ *     allocation on the current device); sizes are element counts;
+ *   - psx_stream is a hipStream_t (NULL = the default stream); kernel entry
+ *     points enqueue work and return without synchronising unless stated;
+ *   - no entry point below the "allocating" section allocates device memory.
+ */
+#ifndef PSX_H
+#define PSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Layout-identical to cufftComplex (interleaved float re, im). */
+typedef struct psx_c32 {
+  float x, y;
+} psx_c32;
+
+typedef void* psx_stream; /* hipStream_t */
+typedef void* psx_event;  /* hipEvent_t */
+
+enum {
+  PSX_OK = 0,
+  PSX_ERR_ARG = 1,         /* bad argument (size, null pointer, range) */
+  PSX_ERR_HIP = 2,         /* HIP runtime error */
+  PSX_ERR_ALLOC = 3,       /* device allocation failed */
+  PSX_ERR_OVERFLOW = 4,    /* output capacity exceeded (result truncated) */
+  PSX_ERR_UNSUPPORTED = 5, /* size or option not supported */
+  PSX_ERR_BUDGET = 6,      /* request exceeds the memory budget */
+  PSX_ERR_STATE = 7,       /* object in the wrong state */
+  PSX_ERR_TIMEOUT = 8      /* a bounded device wait gave up */
+};
+
+/* Error text of the last failing call on this thread (ErrorChecker,
+ * include/utils/exceptions.hpp:13-74, which throws instead). */
+const char* psx_last_error_string(void);
+int psx_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Device, memory and stream plumbing (utils.hpp:23-63 Utils::*)        */
+/* ------------------------------------------------------------------ */
+int psx_device_count(int* count);
+int psx_set_device(int device);
+int psx_get_device(int* device);
+/* name (NUL-terminated, truncated to cap), HBM bytes, compute units. */
+int psx_device_info(int device, char* name, size_t cap, uint64_t* hbm_bytes, int* cu_count, int* arch_gfx);
+int psx_malloc(void** ptr, uint64_t bytes);         /* Utils::device_malloc, utils.hpp:23-27 */
+int psx_free(void* ptr);                             /* Utils::device_free, utils.hpp:35-39 */
+int psx_host_malloc(void** ptr, uint64_t bytes);    /* Utils::host_malloc (pinned), utils.hpp:29-33 */
+int psx_host_free(void* ptr);                        /* Utils::host_free, utils.hpp:41-45 */
+int psx_memcpy_h2d(void* dst, const void* src, uint64_t bytes, psx_stream s); /* utils.hpp:47-51 */
+int psx_memcpy_d2h(void* dst, const void* src, uint64_t bytes, psx_stream s); /* utils.hpp:53-57 */
+int psx_memcpy_d2d(void* dst, const void* src, uint64_t bytes, psx_stream s); /* utils.hpp:59-63 */
+int psx_stream_create(psx_stream* s);
+int psx_stream_destroy(psx_stream s);
+int psx_stream_synchronize(psx_stream s);
+int psx_device_synchronize(void);
+int psx_event_create(psx_event* e);
+int psx_event_destroy(psx_event e);
+int psx_event_record(psx_event e, psx_stream s);
+int psx_event_synchronize(psx_event e);
+int psx_event_elapsed_ms(psx_event start, psx_event stop, float* ms);
+int psx_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* ------------------------------------------------------------------ */
+/* Per-stage kernels (the reference kernel layer, kernels.h:104-296)   */
+/* ------------------------------------------------------------------ */
+
+/* TimeDomainResampler::resampleII (resampler.hpp:26-31) ->
+ * device_resampleII (src/kernels.cu:355-369): out[i] = in[rn(i + i*af*(i-n))],
+ * af = double(float(acc*tsamp))/(2c).  Requires |af|*n < 1 (PSX_ERR_ARG). */
+int psx_resample(const float* in, float* out, uint64_t n, float acc, float tsamp, psx_stream s);
+/* TimeDomainResampler::resample (resampler.hpp:20-24) -> device_resample
+ * (src/kernels.cu:371-386): out[i] = in[rn(i + af*((i-n/2)^2 - (n/2)^2))]. */
+int psx_resample_v1(const float* in, float* out, uint64_t n, float acc, float tsamp, psx_stream s);
+
+/* SpectrumFormer::form (spectrumformer.hpp:16-22) -> device_form_power_series
+ * way 0 (src/kernels.cu:216-234,294-311): A[k] = |X[k]|. */
+int psx_form_amplitude(const psx_c32* X, float* A, uint64_t nb, psx_stream s);
+/* SpectrumFormer::form_interpolated (spectrumformer.hpp:8-14) -> way 1
+ * (src/kernels.cu:238-259): A[k] = sqrt(max(|X_k|^2, |X_k - X_{k-1}|^2/2)). */
+int psx_form_interbin(const psx_c32* X, float* A, uint64_t nb, psx_stream s);
+
+/* median_scrunch5 (src/kernels.cu:974-1008, kernels.h:280-282). */
+int psx_median_scrunch5(const float* in, uint64_t count, float* out, psx_stream s);
+/* linear_stretch (src/kernels.cu:1026-1038, kernels.h:284-287). */
+int psx_linear_stretch(const float* in, uint64_t in_count, float* out, uint64_t out_count, psx_stream s);
+/* device_divide_c_by_f (src/kernels.cu:1040-1061, kernels.h:290-294). */
+int psx_divide_c_by_f(psx_c32* X, const float* f, uint64_t nb, psx_stream s);
+/* Dereddener::calculate_median (dereddener.hpp:41-62): running median of A
+ * into median (nb), using work (>= psx_deredden_work_floats(nb) floats). */
+uint64_t psx_deredden_work_floats(uint64_t nb);
+int psx_deredden_median(const float* A, uint64_t nb, double bin_width, float boundary_5_freq,
+                        float boundary_25_freq, float* median, float* work, psx_stream s);
+
+/* GPU_mean / GPU_rms (src/kernels.cu:427-461) as used by stats::stats
+ * (stats.hpp:25-33).  Synchronises s; results on the host.  work must hold
+ * psx_reduce_work_bytes() bytes of device memory. */
+uint64_t psx_reduce_work_bytes(void);
+int psx_mean_rms(const float* x, uint64_t n, uint64_t first, float* mean, float* rms, void* work,
+                 psx_stream s);
+/* device_normalise (src/kernels.cu:496-521): p = (p - mean)/sigma. */
+int psx_normalise(float* p, float mean, float sigma, uint64_t n, psx_stream s);
+/* GPU_remove_baseline (src/kernels.cu:463-479); synchronises s per pass. */
+int psx_remove_baseline(float* x, uint64_t n, void* work, psx_stream s);
+/* GPU_fill (src/kernels.cu:482-488): x[start, end) = value. */
+int psx_fill(float* x, uint64_t start, uint64_t end, float value, psx_stream s);
+/* device_conversion (src/kernels.cu:1184-1200): out[i] = (float)in[i]. */
+enum { PSX_U8 = 0, PSX_I8 = 1, PSX_U32 = 2, PSX_F32 = 3 };
+int psx_convert_to_f32(const void* in, int in_type, float* out, uint64_t n, psx_stream s);
+
+/* HarmonicFolder::fold (harmonicfolder.hpp:26-38) -> device_harmonic_sum
+ * (src/kernels.cu:34-100,200-209).  folds is a HOST array of nfolds device
+ * pointers (1 <= nfolds <= 5); no pointer table is copied to the device. */
+int psx_harmonic_sum(const float* P, float* const* folds, uint32_t nfolds, uint64_t nb, psx_stream s);
+
+/* device_find_peaks (src/kernels.cu:391-423): stable (ascending-index)
+ * compaction of (i, p[i]) with p[i] > thr over i in [start, end).  Writes at
+ * most cap pairs to d_idx/d_snr and the full count to *d_count (device).
+ * work must hold psx_find_peaks_work_bytes(end - start) bytes. */
+uint64_t psx_find_peaks_work_bytes(uint64_t n);
+int psx_find_peaks(const float* p, int64_t start, int64_t end, float thr, int32_t* d_idx, float* d_snr,
+                   uint32_t cap, uint32_t* d_count, void* work, psx_stream s);
+
+/* ------------------------------------------------------------------ */
+/* FFT (CuFFTerR2C / CuFFTerC2R, include/transforms/ffter.hpp:54-110)   */
+/* ------------------------------------------------------------------ */
+typedef struct psx_fft_plan psx_fft_plan;
+enum { PSX_FFT_R2C = 0, PSX_FFT_C2R = 1 };
+/* Allocating: the plan owns its twiddle tables and work buffer.  n is the real
+ * length (even; 2^k or 3*2^k, 2^4 <= n <= 2^28). */
+int psx_fft_plan_create(uint64_t n, int kind, uint32_t batch, psx_fft_plan** plan);
+int psx_fft_plan_destroy(psx_fft_plan* plan);
+/* R2C: in = batch x n float, out = batch x (n/2+1) psx_c32 (unnormalised).
+ * C2R: in = batch x (n/2+1) psx_c32, out = batch x n float (scaled by n).
+ * in and out may not alias. */
+int psx_fft_exec(psx_fft_plan* plan, const void* in, void* out, psx_stream s);
+/* Number of kernel launches one exec enqueues (for profiling). */
+int psx_fft_plan_describe(const psx_fft_plan* plan, char* buf, size_t cap);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic dedispersed series (deterministic, counter-based RNG)      */
+/* ------------------------------------------------------------------ */
+typedef struct psx_pulsar {
+  double period;   /* s */
+  double accel;    /* m/s^2 */
+  double snr;      /* target folded S/N */
+  double phase0;   /* turns */
+  double duty;     /* FWHM as a fraction of the period */
+} psx_pulsar;
+/* out[i] = N(0,1) (Philox4x32-10, key (seed, trial)) + red noise (random
+ * walk, red_amp x white amplitude at 0.1 Hz; 0 disables) + pulsars, minus the
+ * mean.  work must hold psx_synth_work_bytes(n) bytes. */
+uint64_t psx_synth_work_bytes(uint64_t n);
+int psx_synth_series(float* out, uint64_t n, double tsamp, uint64_t seed, uint32_t trial, double red_amp,
+                     const psx_pulsar* pulsars, int npulsars, void* work, psx_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSX_H */

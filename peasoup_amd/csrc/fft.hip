@@ -1,0 +1,613 @@
+// fft.hip - batched FP32 R2C / C2R FFT for gfx950 (replaces cuFFT behind
+// CuFFTerR2C / CuFFTerC2R, include/transforms/ffter.hpp:54-110).
+//
+// Real length n = 2M is computed as an M-point complex FFT of
+// z[m] = x[2m] + i x[2m+1] followed by an in-place "untangle" pass that
+// yields the n/2+1 non-redundant bins; C2R runs the inverse steps.  The
+// complex FFT is a 1-, 2- or 3-pass four-step decomposition (M = M1*M2 or
+// Q*M2*M1); every pass is one launch of k_fft_pass, which reads its columns
+// once, runs all Stockham stages through LDS, applies the inter-pass twiddle
+// and writes once, so a pass costs one read and one write of the array.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "fft_dev.hpp"
+#include "fft_plan.hpp"
+
+using psx::as_stream;
+using namespace psx::fft;
+
+namespace {
+
+// ------------------------------------------------------------ pass kernel
+template <int S, int L, int C>
+__device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds, const PassArgs& a, float2* out,
+                                 unsigned col0, int t) {
+  constexpr int P = points_per_thread(L);
+  constexpr int NT = C * L / P;
+  constexpr int NSTG = num_stages(L);
+  constexpr int R = radix_at(L, S);
+  constexpr int NB = P / R;
+  constexpr int Ns = ns_before(L, S);
+  constexpr bool LAST = (S == NSTG - 1);
+  const bool cf = (S == 0) ? (a.in_cf != 0) : (LAST ? (a.out_cf != 0) : false);
+
+  // twiddle + butterfly
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    int c, j;
+    bf_coords<L, C, R, NT>(b, t, cf, c, j);
+    if (S > 0) {
+      const int k = j % Ns;
+      constexpr int TS = L / (Ns * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], a.twL[k * r * TS]);
+    }
+    dft<R>(&v[b * R]);
+  }
+
+  if constexpr (!LAST) {
+    constexpr int R2 = radix_at(L, S + 1);
+    constexpr int NB2 = P / R2;
+    if (S > 0) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int c, j;
+      bf_coords<L, C, R, NT>(b, t, cf, c, j);
+      const int k = j % Ns;
+      const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[lds_index<L, C>(c, idxD + r * Ns)] = v[b * R + r];
+    }
+    __syncthreads();
+    const bool cf2 = (S + 1 == NSTG - 1) ? (a.out_cf != 0) : false;
+#pragma unroll
+    for (int b = 0; b < NB2; ++b) {
+      int c, j;
+      bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
+#pragma unroll
+      for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds[lds_index<L, C>(c, j + r * (L / R2))];
+    }
+    run_stage<S + 1, L, C>(v, lds, a, out, col0, t);
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int c, j;
+      bf_coords<L, C, R, NT>(b, t, cf, c, j);
+      const uint64_t cg = (uint64_t)col0 + (uint64_t)c;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int e = j + r * (L / R);
+        float2 val = v[b * R + r];
+        if (a.tw4) val = cmul(val, tw4_lookup(a, cg * (uint64_t)e));
+        if (a.out_conj) val.y = -val.y;
+        out[(int64_t)cg * a.out_col + (int64_t)e * a.out_elem] = val;
+      }
+    }
+  }
+}
+
+template <int L, int C>
+__global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const PassArgs a) {
+  constexpr int P = points_per_thread(L);
+  constexpr int NT = C * L / P;
+  constexpr int R0 = radix_at(L, 0);
+  constexpr int NB0 = P / R0;
+  extern __shared__ float2 lds[];
+  const unsigned blk = psx::xcd_logical_block(blockIdx.x, gridDim.x);
+  const unsigned col0 = blk * C;
+  const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)blockIdx.z * a.in_tr;
+  float2* out = a.out + (int64_t)blockIdx.y * a.out_sub + (int64_t)blockIdx.z * a.out_tr;
+  const int t = threadIdx.x;
+  const bool cf = a.in_cf != 0;
+  float2 v[P];
+#pragma unroll
+  for (int b = 0; b < NB0; ++b) {
+    int c, j;
+    bf_coords<L, C, R0, NT>(b, t, cf, c, j);
+    const int64_t cg = (int64_t)col0 + c;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      float2 x = in[cg * a.in_col + (int64_t)(j + r * (L / R0)) * a.in_elem];
+      if (a.in_conj) x.y = -x.y;
+      v[b * R0 + r] = x;
+    }
+  }
+  run_stage<0, L, C>(v, lds, a, out, col0, t);
+}
+
+// --------------------------------------------------------- R2C untangle
+// In place on the M+1 bins of X: thread k in [0, M/2] reads Z[k], Z[M-k] and
+// writes X[k], X[M-k] with X[k] = (Zk + Zm*)/2 - i W_n^k (Zk - Zm*)/2.
+struct RootTable {
+  const float2* lo;
+  const float2* hi;
+  uint32_t shift, mask;
+};
+
+__device__ inline float2 root(const RootTable& t, uint64_t k) {
+  return cmul(t.hi[k >> t.shift], t.lo[k & t.mask]);
+}
+
+__device__ inline float2 untangle_one(float2 zk, float2 zm, float2 w) {
+  // E = (zk + conj(zm))/2, D = zk - conj(zm), X = E - (i/2) w D
+  const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+  const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+  const float2 wd = cmul(w, d);
+  return make_float2(e.x + 0.5f * wd.y, e.y - 0.5f * wd.x);
+}
+
+__global__ void __launch_bounds__(256) k_untangle(float2* __restrict__ X, uint64_t M, int64_t tr_stride,
+                                                  RootTable rt) {
+  float2* x = X + (int64_t)blockIdx.y * tr_stride;
+  const uint64_t half = M / 2;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= half; k += stride) {
+    const uint64_t km = (k == 0) ? 0 : M - k;
+    const float2 zk = x[k];
+    const float2 zm = x[km];
+    const float2 wk = root(rt, k);
+    const float2 xk = untangle_one(zk, zm, wk);
+    if (k == 0) {
+      // X[M] from Z[0] with W_n^M = -1
+      const float2 xm = untangle_one(zk, zk, make_float2(-1.f, 0.f));
+      x[0] = xk;
+      x[M] = xm;
+    } else if (k == half && (M & 1) == 0) {
+      x[k] = xk;
+    } else {
+      // W_n^(M-k) = -conj(W_n^k)
+      const float2 wm = make_float2(-wk.x, wk.y);
+      const float2 xm = untangle_one(zm, zk, wm);
+      x[k] = xk;
+      x[km] = xm;
+    }
+  }
+}
+
+// C2R pre-pass, out of place: W[k] = conj( (Xk + Xm*) + i conj(W_n^k) (Xk - Xm*) )
+// for k in [0, M), with Xm = X[M-k].  The conjugate feeds the forward passes.
+__device__ inline float2 tangle_one(float2 xk, float2 xm, float2 w) {
+  const float2 s = make_float2(xk.x + xm.x, xk.y - xm.y);
+  const float2 d = make_float2(xk.x - xm.x, xk.y + xm.y);
+  const float2 cd = cmul(make_float2(w.x, -w.y), d);
+  const float2 z = make_float2(s.x - cd.y, s.y + cd.x);
+  return make_float2(z.x, -z.y);
+}
+
+__global__ void __launch_bounds__(256) k_tangle(const float2* __restrict__ X, float2* __restrict__ W, uint64_t M,
+                                                int64_t in_tr, int64_t out_tr, RootTable rt) {
+  const float2* x = X + (int64_t)blockIdx.y * in_tr;
+  float2* w = W + (int64_t)blockIdx.y * out_tr;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += stride)
+    w[k] = tangle_one(x[k], x[M - k], root(rt, k));
+}
+
+// ------------------------------------------------- kernel registry
+struct KernelInfo {
+  int L, C;
+  void (*fn)(const PassArgs);
+  int threads;
+  int lds;
+};
+
+template <int L, int C>
+constexpr KernelInfo kinfo() {
+  return KernelInfo{L, C, &k_fft_pass<L, C>, C * L / points_per_thread(L), lds_bytes(L, C)};
+}
+
+const KernelInfo kKernels[] = {
+    // single-pass (C = 1)
+    kinfo<8, 1>(), kinfo<16, 1>(), kinfo<32, 1>(), kinfo<64, 1>(), kinfo<128, 1>(), kinfo<256, 1>(),
+    kinfo<512, 1>(), kinfo<1024, 1>(), kinfo<2048, 1>(), kinfo<4096, 1>(),
+    kinfo<6, 1>(), kinfo<12, 1>(), kinfo<24, 1>(), kinfo<48, 1>(), kinfo<96, 1>(), kinfo<192, 1>(),
+    kinfo<384, 1>(), kinfo<768, 1>(), kinfo<1536, 1>(), kinfo<3072, 1>(),
+    // multi-pass column tiles
+    kinfo<32, 16>(), kinfo<64, 16>(), kinfo<128, 16>(), kinfo<256, 16>(), kinfo<512, 16>(), kinfo<1024, 16>(),
+    kinfo<64, 8>(), kinfo<128, 8>(), kinfo<256, 8>(), kinfo<512, 8>(), kinfo<1024, 8>(), kinfo<2048, 8>(),
+    kinfo<256, 4>(), kinfo<512, 4>(), kinfo<1024, 4>(), kinfo<2048, 4>(), kinfo<4096, 4>(),
+    kinfo<1024, 2>(), kinfo<2048, 2>(), kinfo<4096, 2>(),
+    kinfo<48, 16>(), kinfo<96, 16>(), kinfo<192, 16>(), kinfo<384, 16>(), kinfo<768, 16>(),
+    kinfo<96, 8>(), kinfo<192, 8>(), kinfo<384, 8>(), kinfo<768, 8>(), kinfo<1536, 8>(),
+    kinfo<384, 4>(), kinfo<768, 4>(), kinfo<1536, 4>(), kinfo<3072, 4>(),
+};
+
+const KernelInfo* find_kernel(int L, int C) {
+  for (const KernelInfo& k : kKernels)
+    if (k.L == L && k.C == C) return &k;
+  return nullptr;
+}
+
+bool g_lds_configured = false;
+
+int configure_lds() {
+  if (g_lds_configured) return PSX_OK;
+  for (const KernelInfo& k : kKernels) {
+    if (k.lds > 65536) {
+      hipError_t e = hipFuncSetAttribute((const void*)k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, k.lds);
+      if (e != hipSuccess)
+        return psx::fail(PSX_ERR_HIP, "hipFuncSetAttribute(L=%d,C=%d,lds=%d): %s", k.L, k.C, k.lds,
+                         hipGetErrorString(e));
+    }
+  }
+  g_lds_configured = true;
+  return PSX_OK;
+}
+
+// ------------------------------------------------------- host tables
+void root_table_host(uint64_t m, uint32_t shift, std::vector<float2>& lo, std::vector<float2>& hi) {
+  // W_m^x = W_m^(hi part) * W_m^(lo part), x = (h << shift) + l
+  const uint64_t T = 1ull << shift;
+  lo.resize(T);
+  hi.resize((m + T - 1) / T + 1);
+  const long double tp = -2.0L * 3.14159265358979323846264338327950288L / (long double)m;
+  for (uint64_t l = 0; l < T; ++l) {
+    long double a = tp * (long double)l;
+    lo[l] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  for (uint64_t h = 0; h < hi.size(); ++h) {
+    long double a = tp * (long double)(h * T);
+    hi[h] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+}
+
+void twl_host(int L, std::vector<float2>& t) {
+  t.resize(L);
+  const long double tp = -2.0L * 3.14159265358979323846264338327950288L / (long double)L;
+  for (int e = 0; e < L; ++e) t[e] = make_float2((float)cosl(tp * e), (float)sinl(tp * e));
+}
+
+}  // namespace
+
+// =================================================== plan
+namespace psx {
+namespace fft {
+
+bool valid_complex_len(uint64_t m) {
+  if (m < 3) return false;
+  uint64_t p = m % 3 == 0 ? m / 3 : m;
+  return (p & (p - 1)) == 0;
+}
+
+static bool len_supported(uint64_t L, bool multi) {
+  for (const KernelInfo& k : kKernels)
+    if ((uint64_t)k.L == L && (multi ? k.C > 1 : k.C == 1)) return true;
+  return false;
+}
+
+// Default decomposition: one pass when a single-workgroup kernel exists,
+// else the two-pass split with the smallest largest factor (M1 >= M2 on a
+// tie), else the most balanced three-pass split.
+bool default_decomp(uint64_t M, Decomp& d) {
+  d = Decomp();
+  if (!valid_complex_len(M)) return false;
+  if (len_supported(M, false)) {
+    d.npass = 1;
+    d.L[0] = (int)M;
+    d.C[0] = 1;
+    return true;
+  }
+  uint64_t best = 0, bM1 = 0, bM2 = 0;
+  for (uint64_t M1 = 32; M1 <= 4096 && M1 < M; M1 = (M1 % 3 == 0) ? (M1 / 3) * 4 : M1 + M1 / 2) {
+    if (M % M1) continue;
+    const uint64_t M2 = M / M1;
+    if (!len_supported(M1, true) || !len_supported(M2, true)) continue;
+    const uint64_t score = M1 > M2 ? M1 : M2;
+    if (!best || score < best || (score == best && M1 > bM1)) {
+      best = score;
+      bM1 = M1;
+      bM2 = M2;
+    }
+  }
+  if (best) {
+    d.npass = 2;
+    d.L[0] = (int)bM2;
+    d.L[1] = (int)bM1;
+    return pick_tiles(d);
+  }
+  uint64_t bQ = 0;
+  best = 0;
+  for (uint64_t M1 = 32; M1 <= 4096; M1 = (M1 % 3 == 0) ? (M1 / 3) * 4 : M1 + M1 / 2) {
+    if (M % M1 || !len_supported(M1, true)) continue;
+    for (uint64_t M2 = 32; M2 <= 4096; M2 = (M2 % 3 == 0) ? (M2 / 3) * 4 : M2 + M2 / 2) {
+      if ((M / M1) % M2 || !len_supported(M2, true)) continue;
+      const uint64_t Q = M / M1 / M2;
+      if (!len_supported(Q, true)) continue;
+      uint64_t score = M1;
+      if (M2 > score) score = M2;
+      if (Q > score) score = Q;
+      if (!best || score < best || (score == best && M1 > bM1)) {
+        best = score;
+        bM1 = M1;
+        bM2 = M2;
+        bQ = Q;
+      }
+    }
+  }
+  if (!best) return false;
+  d.npass = 3;
+  d.L[0] = (int)bQ;
+  d.L[1] = (int)bM2;
+  d.L[2] = (int)bM1;
+  return pick_tiles(d);
+}
+
+bool pick_tiles(Decomp& d) {
+  for (int p = 0; p < d.npass; ++p) {
+    if (d.C[p] > 0 && find_kernel(d.L[p], d.C[p])) continue;
+    const int pref[] = {16, 8, 4, 2};
+    d.C[p] = 0;
+    for (int c : pref) {
+      const KernelInfo* k = find_kernel(d.L[p], c);
+      if (k && k->lds <= 80 * 1024) {
+        d.C[p] = c;
+        break;
+      }
+    }
+    if (!d.C[p])
+      for (int c : pref)
+        if (find_kernel(d.L[p], c)) {
+          d.C[p] = c;
+          break;
+        }
+    if (!d.C[p]) return false;
+  }
+  return true;
+}
+
+bool decomp_supported(const Decomp& d) {
+  for (int p = 0; p < d.npass; ++p)
+    if (!find_kernel(d.L[p], d.C[p])) return false;
+  return true;
+}
+
+}  // namespace fft
+}  // namespace psx
+
+struct PassLaunch {
+  const KernelInfo* k;
+  PassArgs a;           // in/out pointers filled at exec time from src/dst
+  int src, dst;         // 0 = user input, 1 = work, 2 = user output
+  unsigned gx, gy, gz;
+};
+
+struct psx_fft_plan {
+  uint64_t n = 0, M = 0, nb = 0;
+  int kind = 0;
+  uint32_t batch = 1;
+  psx::fft::Decomp d;
+  std::vector<PassLaunch> passes;
+  float2* tables = nullptr;  // all device tables in one allocation
+  float2* work = nullptr;    // M complex per transform (when needed)
+  RootTable rt{};            // W_n roots for (un)tangle
+};
+
+namespace {
+
+int build_passes(psx_fft_plan* p) {
+  using psx::fft::Decomp;
+  const Decomp& d = p->d;
+  const uint64_t M = p->M;
+  // --- device tables: per-pass W_L, four-step roots, W_n roots
+  std::vector<float2> host;
+  std::vector<size_t> twl_off(d.npass), tw4lo_off(d.npass), tw4hi_off(d.npass);
+  std::vector<uint32_t> tw4_shift(d.npass);
+  auto append = [&](const std::vector<float2>& v) {
+    size_t o = host.size();
+    host.insert(host.end(), v.begin(), v.end());
+    while (host.size() % 32) host.push_back(make_float2(0.f, 0.f));
+    return o;
+  };
+  std::vector<float2> tmp, lo, hi;
+  // four-step twiddle sizes per pass
+  uint64_t mtw[3] = {0, 0, 0};
+  if (d.npass == 2) mtw[0] = M;
+  if (d.npass == 3) {
+    mtw[0] = M;
+    mtw[1] = (uint64_t)d.L[1] * (uint64_t)d.L[2];
+  }
+  for (int q = 0; q < d.npass; ++q) {
+    twl_host(d.L[q], tmp);
+    twl_off[q] = append(tmp);
+    if (mtw[q]) {
+      uint32_t sh = 0;
+      while ((1ull << (sh + 1)) * (1ull << (sh + 1)) <= mtw[q] && (mtw[q] % (1ull << (sh + 1))) == 0) ++sh;
+      tw4_shift[q] = sh;
+      root_table_host(mtw[q], sh, lo, hi);
+      tw4lo_off[q] = append(lo);
+      tw4hi_off[q] = append(hi);
+    }
+  }
+  uint32_t rsh = 0;
+  const uint64_t n = p->n;
+  while ((1ull << (rsh + 1)) * (1ull << (rsh + 1)) <= n && (n % (1ull << (rsh + 1))) == 0) ++rsh;
+  root_table_host(n, rsh, lo, hi);
+  const size_t rlo = append(lo), rhi = append(hi);
+  PSX_CHECK_HIP(hipMalloc(&p->tables, host.size() * sizeof(float2)));
+  PSX_CHECK_HIP(hipMemcpy(p->tables, host.data(), host.size() * sizeof(float2), hipMemcpyHostToDevice));
+  p->rt = RootTable{p->tables + rlo, p->tables + rhi, rsh, (uint32_t)((1ull << rsh) - 1)};
+
+  // --- passes
+  const bool c2r = p->kind == PSX_FFT_C2R;
+  const int64_t trM = (int64_t)M;                // complex stride of work / real-as-complex buffers
+  const int64_t trNB = (int64_t)(M + 1);         // complex stride of spectra
+  const int64_t in_tr = c2r ? trM : trM;         // passes read z (R2C: input as complex) or work
+  auto make = [&](int q, int src, int dst) {
+    PassLaunch L{};
+    L.k = find_kernel(d.L[q], d.C[q]);
+    L.src = src;
+    L.dst = dst;
+    PassArgs& a = L.a;
+    a.twL = p->tables + twl_off[q];
+    if (mtw[q]) {
+      a.tw4 = 1;
+      a.tw4_lo = p->tables + tw4lo_off[q];
+      a.tw4_hi = p->tables + tw4hi_off[q];
+      a.tw4_shift = tw4_shift[q];
+      a.tw4_mask = (uint32_t)((1ull << tw4_shift[q]) - 1);
+    }
+    a.in_tr = (src == 2) ? trNB : in_tr;
+    a.out_tr = (dst == 2) ? (c2r ? trM : trNB) : trM;
+    return L;
+  };
+  if (d.npass == 1) {
+    PassLaunch L = make(0, c2r ? 1 : 0, 2);
+    L.a.in_col = 0; L.a.in_elem = 1; L.a.in_sub = 0;
+    L.a.out_col = 0; L.a.out_elem = 1; L.a.out_sub = 0;
+    L.gx = 1; L.gy = 1; L.gz = p->batch;
+    p->passes.push_back(L);
+  } else if (d.npass == 2) {
+    const int64_t M2 = d.L[0], M1 = d.L[1];
+    PassLaunch A = make(0, c2r ? 1 : 0, 1);
+    A.a.in_col = 1; A.a.in_elem = M1; A.a.in_sub = 0;
+    A.a.out_col = 1; A.a.out_elem = M1; A.a.out_sub = 0;
+    A.a.in_cf = 1; A.a.out_cf = 1;
+    A.gx = (unsigned)(M1 / d.C[0]); A.gy = 1; A.gz = p->batch;
+    PassLaunch B = make(1, 1, 2);
+    B.a.in_col = M1; B.a.in_elem = 1; B.a.in_sub = 0;
+    B.a.out_col = 1; B.a.out_elem = M2; B.a.out_sub = 0;
+    B.a.in_cf = 0; B.a.out_cf = 1;
+    B.gx = (unsigned)(M2 / d.C[1]); B.gy = 1; B.gz = p->batch;
+    p->passes.push_back(A);
+    p->passes.push_back(B);
+  } else {
+    const int64_t Q = d.L[0], M2 = d.L[1], M1 = d.L[2], P = M1 * M2;
+    PassLaunch A = make(0, c2r ? 1 : 0, 1);
+    A.a.in_col = 1; A.a.in_elem = P; A.a.in_sub = 0;
+    A.a.out_col = 1; A.a.out_elem = P; A.a.out_sub = 0;
+    A.a.in_cf = 1; A.a.out_cf = 1;
+    A.gx = (unsigned)(P / d.C[0]); A.gy = 1; A.gz = p->batch;
+    PassLaunch B = make(1, 1, 1);
+    B.a.in_col = 1; B.a.in_elem = M1; B.a.in_sub = P;
+    B.a.out_col = 1; B.a.out_elem = M1; B.a.out_sub = P;
+    B.a.in_cf = 1; B.a.out_cf = 1;
+    B.gx = (unsigned)(M1 / d.C[1]); B.gy = (unsigned)Q; B.gz = p->batch;
+    PassLaunch Cc = make(2, 1, 2);
+    Cc.a.in_col = P; Cc.a.in_elem = 1; Cc.a.in_sub = M1;
+    Cc.a.out_col = 1; Cc.a.out_elem = Q * M2; Cc.a.out_sub = Q;
+    Cc.a.in_cf = 0; Cc.a.out_cf = 1;
+    Cc.gx = (unsigned)(Q / d.C[2]); Cc.gy = (unsigned)M2; Cc.gz = p->batch;
+    p->passes.push_back(A);
+    p->passes.push_back(B);
+    p->passes.push_back(Cc);
+  }
+  if (c2r) p->passes.back().a.out_conj = 1;
+  for (const PassLaunch& L : p->passes) {
+    if (!L.k) return psx::fail(PSX_ERR_UNSUPPORTED, "no FFT kernel for this decomposition");
+    if (L.gx < 1 || L.gy > 65535 || L.gz > 65535) return psx::fail(PSX_ERR_UNSUPPORTED, "FFT grid too large");
+  }
+  return PSX_OK;
+}
+
+}  // namespace
+
+namespace psx {
+namespace fft {
+
+int plan_create_with(uint64_t n, int kind, uint32_t batch, const Decomp* forced, psx_fft_plan** out) {
+  PSX_REQUIRE(out, "psx_fft_plan_create: null plan");
+  *out = nullptr;
+  PSX_REQUIRE(kind == PSX_FFT_R2C || kind == PSX_FFT_C2R, "psx_fft_plan_create: bad kind %d", kind);
+  PSX_REQUIRE(n >= 16 && n <= (1ull << 28) && (n % 2) == 0 && valid_complex_len(n / 2),
+              "psx_fft_plan_create: unsupported length %llu (need 2^k or 3*2^k, 16 <= n <= 2^28)",
+              (unsigned long long)n);
+  PSX_REQUIRE(batch >= 1 && batch <= 65535, "psx_fft_plan_create: batch must be 1..65535");
+  int rc = configure_lds();
+  if (rc) return rc;
+  psx_fft_plan* p = new psx_fft_plan;
+  p->n = n;
+  p->M = n / 2;
+  p->nb = n / 2 + 1;
+  p->kind = kind;
+  p->batch = batch;
+  bool okd = forced ? (p->d = *forced, pick_tiles(p->d)) : default_decomp(p->M, p->d);
+  if (!okd || !decomp_supported(p->d)) {
+    delete p;
+    return psx::fail(PSX_ERR_UNSUPPORTED, "psx_fft_plan_create: no decomposition for n=%llu",
+                     (unsigned long long)n);
+  }
+  uint64_t prod = 1;
+  for (int q = 0; q < p->d.npass; ++q) prod *= (uint64_t)p->d.L[q];
+  if (prod != p->M) {
+    delete p;
+    return psx::fail(PSX_ERR_ARG, "psx_fft_plan_create: decomposition does not multiply to M");
+  }
+  rc = build_passes(p);
+  if (!rc) {
+    const bool need_work = (kind == PSX_FFT_C2R) || p->d.npass > 1;
+    if (need_work) {
+      hipError_t e = hipMalloc(&p->work, sizeof(float2) * p->M * batch);
+      if (e != hipSuccess) rc = psx::fail(PSX_ERR_ALLOC, "FFT work buffer: %s", hipGetErrorString(e));
+    }
+  }
+  if (rc) {
+    psx_fft_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return psx::ok();
+}
+
+int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t hs) {
+  const unsigned tb = 256;
+  if (p->kind == PSX_FFT_C2R) {
+    const unsigned g = psx::grid_for(p->M, tb);
+    k_tangle<<<dim3(g, p->batch), tb, 0, hs>>>((const float2*)in, p->work, p->M, (int64_t)p->nb, (int64_t)p->M,
+                                               p->rt);
+    PSX_CHECK_LAUNCH("k_tangle");
+  }
+  for (PassLaunch& L : p->passes) {
+    PassArgs a = L.a;
+    a.in = L.src == 0 ? (const float2*)in : (L.src == 1 ? p->work : (const float2*)out);
+    a.out = L.dst == 2 ? (float2*)out : p->work;
+    hipLaunchKernelGGL(L.k->fn, dim3(L.gx, L.gy, L.gz), dim3(L.k->threads), L.k->lds, hs, a);
+    PSX_CHECK_LAUNCH("k_fft_pass");
+  }
+  if (p->kind == PSX_FFT_R2C) {
+    const unsigned g = psx::grid_for(p->M / 2 + 1, tb);
+    k_untangle<<<dim3(g, p->batch), tb, 0, hs>>>((float2*)out, p->M, (int64_t)p->nb, p->rt);
+    PSX_CHECK_LAUNCH("k_untangle");
+  }
+  return PSX_OK;
+}
+
+}  // namespace fft
+}  // namespace psx
+
+extern "C" int psx_fft_plan_create(uint64_t n, int kind, uint32_t batch, psx_fft_plan** plan) {
+  return psx::fft::plan_create_with(n, kind, batch, nullptr, plan);
+}
+
+extern "C" int psx_fft_plan_destroy(psx_fft_plan* plan) {
+  if (!plan) return psx::ok();
+  if (plan->tables) (void)hipFree(plan->tables);
+  if (plan->work) (void)hipFree(plan->work);
+  delete plan;
+  return psx::ok();
+}
+
+extern "C" int psx_fft_exec(psx_fft_plan* plan, const void* in, void* out, psx_stream s) {
+  PSX_REQUIRE(plan && in && out, "psx_fft_exec: null argument");
+  PSX_REQUIRE(in != out, "psx_fft_exec: in-place transforms are not supported");
+  PSX_REQUIRE(((uintptr_t)in & 7) == 0 && ((uintptr_t)out & 7) == 0, "psx_fft_exec: buffers must be 8-byte aligned");
+  int rc = psx::fft::plan_exec(plan, in, out, as_stream(s));
+  if (rc) return rc;
+  return psx::ok();
+}
+
+extern "C" int psx_fft_plan_describe(const psx_fft_plan* p, char* buf, size_t cap) {
+  PSX_REQUIRE(p && buf && cap, "psx_fft_plan_describe: bad arguments");
+  std::string s = "n=" + std::to_string(p->n) + (p->kind == PSX_FFT_R2C ? " R2C" : " C2R") +
+                  " batch=" + std::to_string(p->batch) + " passes=";
+  for (int q = 0; q < p->d.npass; ++q)
+    s += (q ? "x" : "") + std::to_string(p->d.L[q]) + "/C" + std::to_string(p->d.C[q]);
+  s += " launches=" + std::to_string(p->passes.size() + 1);
+  strncpy(buf, s.c_str(), cap - 1);
+  buf[cap - 1] = 0;
+  return psx::ok();
+}

@@ -1,0 +1,206 @@
+// fft_dev.hpp - device building blocks of the FP32 Stockham FFT (gfx950).
+//
+// A "pass" transforms C columns of length L per workgroup.  Column c,
+// element e lives at  base + c*col_stride + e*elem_stride  in global memory;
+// the transform runs as radix-R Stockham autosort stages (R in {16,8,4,2,3})
+// whose intermediate results go through LDS.  Each thread owns P points in
+// registers; stage s assigns butterfly slot b of thread t to
+// g = b*NT + t, mapped either column-fast (c = g % C) for the global side
+// whose columns are adjacent in memory, or element-fast (j = g % (L/R)).
+// Only forward transforms are instantiated: the inverse is computed as
+// conj(FFT(conj(x))), with the conjugations folded into the neighbouring
+// passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace psx {
+namespace fft {
+
+// ------------------------------------------------------------ radix plans
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+__host__ __device__ constexpr bool has3(int L) { return L % 3 == 0; }
+__host__ __device__ constexpr int pow2part(int L) { return has3(L) ? L / 3 : L; }
+// Radix list: [3?] then as many 16s as fit, then the 2^r remainder.
+__host__ __device__ constexpr int num_stages(int L) {
+  return (has3(L) ? 1 : 0) + ilog2c(pow2part(L)) / 4 + ((ilog2c(pow2part(L)) % 4) ? 1 : 0);
+}
+__host__ __device__ constexpr int radix_at(int L, int s) {
+  return (has3(L) && s == 0) ? 3
+         : ((s - (has3(L) ? 1 : 0)) < ilog2c(pow2part(L)) / 4) ? 16
+                                                                : (1 << (ilog2c(pow2part(L)) % 4));
+}
+__host__ __device__ constexpr int max_pow2_radix(int L) {
+  return ilog2c(pow2part(L)) >= 4 ? 16 : (1 << ilog2c(pow2part(L)));
+}
+// Points per thread: a multiple of every radix of the plan.
+__host__ __device__ constexpr int points_per_thread(int L) {
+  return has3(L) ? 3 * max_pow2_radix(L) : max_pow2_radix(L);
+}
+__host__ __device__ constexpr int ns_before(int L, int s) { return s == 0 ? 1 : ns_before(L, s - 1) * radix_at(L, s - 1); }
+// LDS row pitch in float2: one pad slot per 16 elements plus one per row.
+__host__ __device__ constexpr int lds_pitch(int L, int C) { return L + L / 16 + (C > 1 ? 1 : 0); }
+__host__ __device__ constexpr int lds_bytes(int L, int C) { return C * lds_pitch(L, C) * 8; }
+
+// ---------------------------------------------------------- complex math
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+// a * (-i)
+__device__ inline float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// Forward DFTs (exp(-2 pi i rk/R)), in place, natural order in and out.
+__device__ inline void dft2(float2& a, float2& b) {
+  const float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+__device__ inline void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+  const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = mul_mi(csub(v1, v3));
+  v0 = cadd(a0, a2);
+  v2 = csub(a0, a2);
+  v1 = cadd(a1, a3);
+  v3 = csub(a1, a3);
+}
+
+__device__ inline void dft3(float2& v0, float2& v1, float2& v2) {
+  const float s3 = 0.866025403784438646763723170752936f;
+  const float2 t1 = cadd(v1, v2), t2 = csub(v1, v2);
+  const float2 base = make_float2(v0.x - 0.5f * t1.x, v0.y - 0.5f * t1.y);
+  // -i * s3 * t2
+  const float2 rot = make_float2(s3 * t2.y, -s3 * t2.x);
+  v0 = cadd(v0, t1);
+  v1 = cadd(base, rot);
+  v2 = csub(base, rot);
+}
+
+// W_16^m = exp(-2 pi i m/16), m = 0..9 (the products q*k1 of the 4x4 split).
+__device__ inline float2 w16(int m) {
+  const float c1 = 0.923879532511286756128183189396788f;  // cos(pi/8)
+  const float s1 = 0.382683432365089771728459984030399f;  // sin(pi/8)
+  const float h = 0.707106781186547524400844362104849f;
+  switch (m) {
+    case 0: return make_float2(1.f, 0.f);
+    case 1: return make_float2(c1, -s1);
+    case 2: return make_float2(h, -h);
+    case 3: return make_float2(s1, -c1);
+    case 4: return make_float2(0.f, -1.f);
+    case 5: return make_float2(-s1, -c1);
+    case 6: return make_float2(-h, -h);
+    case 7: return make_float2(-c1, -s1);
+    case 8: return make_float2(-1.f, 0.f);
+    default: return make_float2(-c1, s1);  // 9
+  }
+}
+
+// DFT8 = 2 x 4: n = q + 2r, k = k1 + 4 k2.
+__device__ inline void dft8(float2* v) {
+  float2 a[2][4];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    a[q][0] = v[q];
+    a[q][1] = v[q + 2];
+    a[q][2] = v[q + 4];
+    a[q][3] = v[q + 6];
+    dft4(a[q][0], a[q][1], a[q][2], a[q][3]);
+  }
+  const float h = 0.707106781186547524400844362104849f;
+  a[1][1] = cmul(a[1][1], make_float2(h, -h));
+  a[1][2] = mul_mi(a[1][2]);
+  a[1][3] = cmul(a[1][3], make_float2(-h, -h));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 x0 = a[0][k1], x1 = a[1][k1];
+    dft2(x0, x1);
+    v[k1] = x0;
+    v[k1 + 4] = x1;
+  }
+}
+
+// DFT16 = 4 x 4: n = q + 4r, k = k1 + 4 k2, twiddle W16^(q*k1).
+__device__ inline void dft16(float2* v) {
+  float2 a[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    a[q][0] = v[q];
+    a[q][1] = v[q + 4];
+    a[q][2] = v[q + 8];
+    a[q][3] = v[q + 12];
+    dft4(a[q][0], a[q][1], a[q][2], a[q][3]);
+  }
+#pragma unroll
+  for (int q = 1; q < 4; ++q)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) a[q][k1] = (q * k1 == 4) ? mul_mi(a[q][k1]) : cmul(a[q][k1], w16(q * k1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 x0 = a[0][k1], x1 = a[1][k1], x2 = a[2][k1], x3 = a[3][k1];
+    dft4(x0, x1, x2, x3);
+    v[k1] = x0;
+    v[k1 + 4] = x1;
+    v[k1 + 8] = x2;
+    v[k1 + 12] = x3;
+  }
+}
+
+template <int R>
+__device__ inline void dft(float2* v) {
+  if constexpr (R == 2) dft2(v[0], v[1]);
+  else if constexpr (R == 3) dft3(v[0], v[1], v[2]);
+  else if constexpr (R == 4) dft4(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 8) dft8(v);
+  else dft16(v);
+}
+
+// ------------------------------------------------------------ pass args
+struct PassArgs {
+  const float2* in;
+  float2* out;
+  int64_t in_col, in_elem, in_sub, in_tr;
+  int64_t out_col, out_elem, out_sub, out_tr;
+  const float2* twL;     // W_L^e, e in [0, L)
+  const float2* tw4_lo;  // W_Mtw^(x & mask)
+  const float2* tw4_hi;  // W_Mtw^((x >> shift) << shift)
+  uint32_t tw4_shift;
+  uint32_t tw4_mask;
+  int32_t tw4;       // multiply output (c, e) by W_Mtw^(c*e)
+  int32_t in_cf;     // stage-0 mapping column-fast
+  int32_t out_cf;    // last-stage mapping column-fast
+  int32_t in_conj;   // conjugate inputs on load
+  int32_t out_conj;  // conjugate outputs on store
+};
+
+__device__ inline float2 tw4_lookup(const PassArgs& a, uint64_t x) {
+  const float2 lo = a.tw4_lo[x & a.tw4_mask];
+  const float2 hi = a.tw4_hi[x >> a.tw4_shift];
+  return cmul(hi, lo);
+}
+
+template <int L, int C>
+__device__ inline int lds_index(int c, int e) {
+  return c * lds_pitch(L, C) + e + (e >> 4);
+}
+
+template <int L, int C, int R, int NT>
+__device__ inline void bf_coords(int b, int t, bool cf, int& c, int& j) {
+  const int g = b * NT + t;
+  constexpr int NBC = L / R;
+  if (C == 1) {
+    c = 0;
+    j = g;
+  } else if (cf) {
+    c = g % C;
+    j = g / C;
+  } else {
+    c = g / NBC;
+    j = g % NBC;
+  }
+}
+
+}  // namespace fft
+}  // namespace psx
