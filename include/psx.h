@@ -167,6 +167,53 @@ uint64_t psx_synth_work_bytes(uint64_t n);
 int psx_synth_series(float* out, uint64_t n, double tsamp, uint64_t seed, uint32_t trial, double red_amp,
                      const psx_pulsar* pulsars, int npulsars, void* work, psx_stream s);
 
+/* ------------------------------------------------------------------ */
+/* Whole-DM-trial search drivers (Worker level)                        */
+/* ------------------------------------------------------------------ */
+/* The CmdLineOptions fields the reference Worker reads
+ * (include/utils/cmdline.hpp:77-256; pipeline_multi.cu:111-149,412-421). */
+typedef struct psx_search_args {
+  uint64_t size;          /* FFT length N (--fft_size) */
+  float tsamp;            /* sampling time, s */
+  float min_snr;          /* -m/--min_snr */
+  float min_freq;         /* --min_freq, Hz */
+  float max_freq;         /* --max_freq, Hz */
+  uint32_t nharmonics;    /* -n/--nharmonics, 1..5 */
+  float freq_tol;         /* --freq_tol */
+  float max_harm;         /* --max_harm_match */
+  float acc_start;        /* --acc_start, m/s^2 */
+  float acc_end;          /* --acc_end, m/s^2 */
+  float acc_tol;          /* --acc_tol */
+  float acc_pulse_width;  /* --acc_pulse_width, microseconds */
+  float cdm;              /* --cdm */
+  float cfreq;            /* centre frequency, Hz (get_cfreq()*1e6) */
+  float ch_bw;            /* channel bandwidth, Hz (get_foff()*1e6) */
+  int32_t min_gap;        /* PeakFinder min_gap (30) */
+} psx_search_args;
+
+/* CandidatePOD fields (include/data_types/candidates.hpp:10-17) plus the
+ * associated-candidate count (Candidate::count_assoc, :44-51). */
+typedef struct psx_candidate {
+  float dm;
+  int32_t dm_idx;
+  float acc;
+  int32_t nh;
+  float snr;
+  float freq;
+  int32_t nassoc;
+  int32_t reserved;
+} psx_candidate;
+
+/* Worker::start (src/pipeline_multi.cu:111-303) over ndm host series of
+ * nsamps samples each (row-major, DispersionTrials layout), one call per
+ * kernel as the reference does.  accs (naccs values) overrides the
+ * AccelerationPlan list when non-NULL.  Writes the acceleration-distilled
+ * candidates of every DM in DM order (*count = total; PSX_ERR_OVERFLOW if it
+ * exceeds cap).  seconds[0..2] (optional) = prologue, accel loop, distill. */
+int psx_worker_search(const psx_search_args* args, const float* series, uint64_t nsamps, const float* dm_list,
+                      uint32_t ndm, const float* accs, uint32_t naccs, psx_candidate* out, uint32_t cap,
+                      uint32_t* count, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif

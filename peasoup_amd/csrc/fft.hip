@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "fft_dev.hpp"
 #include "fft_plan.hpp"
+#include "kernels_dev.hpp"
 
 using psx::as_stream;
 using namespace psx::fft;
@@ -105,16 +106,34 @@ __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const
   const int t = threadIdx.x;
   const bool cf = a.in_cf != 0;
   float2 v[P];
+  if (a.resample) {
+    // resampleII fused into the first read (src/kernels.cu:342-353)
+    const TrialDesc d = a.descs[blockIdx.z];
+    const int64_t sub = (int64_t)blockIdx.y * a.in_sub;
 #pragma unroll
-  for (int b = 0; b < NB0; ++b) {
-    int c, j;
-    bf_coords<L, C, R0, NT>(b, t, cf, c, j);
-    const int64_t cg = (int64_t)col0 + c;
+    for (int b = 0; b < NB0; ++b) {
+      int c, j;
+      bf_coords<L, C, R0, NT>(b, t, cf, c, j);
+      const int64_t cg = (int64_t)col0 + c;
 #pragma unroll
-    for (int r = 0; r < R0; ++r) {
-      float2 x = in[cg * a.in_col + (int64_t)(j + r * (L / R0)) * a.in_elem];
-      if (a.in_conj) x.y = -x.y;
-      v[b * R0 + r] = x;
+      for (int r = 0; r < R0; ++r) {
+        const uint64_t m = (uint64_t)(sub + cg * a.in_col + (int64_t)(j + r * (L / R0)) * a.in_elem);
+        v[b * R0 + r] = make_float2(d.tim[psx::resample_index(2 * m, a.rs_n, d.af)],
+                                    d.tim[psx::resample_index(2 * m + 1, a.rs_n, d.af)]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB0; ++b) {
+      int c, j;
+      bf_coords<L, C, R0, NT>(b, t, cf, c, j);
+      const int64_t cg = (int64_t)col0 + c;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        float2 x = in[cg * a.in_col + (int64_t)(j + r * (L / R0)) * a.in_elem];
+        if (a.in_conj) x.y = -x.y;
+        v[b * R0 + r] = x;
+      }
     }
   }
   run_stage<0, L, C>(v, lds, a, out, col0, t);
@@ -123,23 +142,6 @@ __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const
 // --------------------------------------------------------- R2C untangle
 // In place on the M+1 bins of X: thread k in [0, M/2] reads Z[k], Z[M-k] and
 // writes X[k], X[M-k] with X[k] = (Zk + Zm*)/2 - i W_n^k (Zk - Zm*)/2.
-struct RootTable {
-  const float2* lo;
-  const float2* hi;
-  uint32_t shift, mask;
-};
-
-__device__ inline float2 root(const RootTable& t, uint64_t k) {
-  return cmul(t.hi[k >> t.shift], t.lo[k & t.mask]);
-}
-
-__device__ inline float2 untangle_one(float2 zk, float2 zm, float2 w) {
-  // E = (zk + conj(zm))/2, D = zk - conj(zm), X = E - (i/2) w D
-  const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-  const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
-  const float2 wd = cmul(w, d);
-  return make_float2(e.x + 0.5f * wd.y, e.y - 0.5f * wd.x);
-}
 
 __global__ void __launch_bounds__(256) k_untangle(float2* __restrict__ X, uint64_t M, int64_t tr_stride,
                                                   RootTable rt) {
@@ -575,6 +577,29 @@ int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t hs) {
   }
   return PSX_OK;
 }
+
+int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs) {
+  if (p->kind != PSX_FFT_R2C || ntrials < 1 || ntrials > p->batch)
+    return psx::fail(PSX_ERR_ARG, "exec_fused_r2c: bad plan or trial count");
+  for (size_t q = 0; q < p->passes.size(); ++q) {
+    PassLaunch& L = p->passes[q];
+    PassArgs a = L.a;
+    a.in = L.src == 1 ? p->work : nullptr;
+    a.out = L.dst == 2 ? zout : p->work;
+    if (q == 0) {
+      a.resample = 1;
+      a.rs_n = p->n;
+      a.descs = d_descs;
+    }
+    hipLaunchKernelGGL(L.k->fn, dim3(L.gx, L.gy, ntrials), dim3(L.k->threads), L.k->lds, hs, a);
+    PSX_CHECK_LAUNCH("k_fft_pass(fused)");
+  }
+  return PSX_OK;
+}
+
+RootTable root_table(const psx_fft_plan* p) { return p->rt; }
+uint64_t complex_len(const psx_fft_plan* p) { return p->M; }
+const Decomp& plan_decomp(const psx_fft_plan* p) { return p->d; }
 
 }  // namespace fft
 }  // namespace psx

@@ -158,6 +158,18 @@ __device__ inline void dft(float2* v) {
 }
 
 // ------------------------------------------------------------ pass args
+// One acceleration trial of a batch: the dereddened series of its DM and the
+// resampling factor af (device_resampleII, src/kernels.cu:361), plus the
+// normalisation constants mean*N, std*N (pipeline_multi.cu:271).
+struct TrialDesc {
+  const float* tim;
+  double af;
+  float mean_n;
+  float std_n;
+  uint32_t slot;
+  uint32_t pad;
+};
+
 struct PassArgs {
   const float2* in;
   float2* out;
@@ -173,6 +185,12 @@ struct PassArgs {
   int32_t out_cf;    // last-stage mapping column-fast
   int32_t in_conj;   // conjugate inputs on load
   int32_t out_conj;  // conjugate outputs on store
+  // Fused resampling (first pass of a batched search): complex input m of
+  // trial blockIdx.z is (tim[idx(2m)], tim[idx(2m+1)]) with the resample index
+  // of the trial's af over a series of rs_n samples.
+  int32_t resample;
+  uint64_t rs_n;
+  const TrialDesc* descs;
 };
 
 __device__ inline float2 tw4_lookup(const PassArgs& a, uint64_t x) {

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "fft_dev.hpp"
 #include "psx.h"
 
 namespace psx {
@@ -23,8 +24,42 @@ bool valid_complex_len(uint64_t m);
 bool default_decomp(uint64_t M, Decomp& d);
 bool pick_tiles(Decomp& d);
 bool decomp_supported(const Decomp& d);
+// Roots W_n^k of the real length n as a two-level table: W = hi[k>>shift]*lo[k&mask].
+struct RootTable {
+  const float2* lo;
+  const float2* hi;
+  uint32_t shift, mask;
+};
+
+__device__ inline float2 root(const RootTable& t, uint64_t k) { return cmul(t.hi[k >> t.shift], t.lo[k & t.mask]); }
+
+// R2C untangle of one bin: X = (zk + conj(zm))/2 - (i/2) w (zk - conj(zm)).
+__device__ inline float2 untangle_one(float2 zk, float2 zm, float2 w) {
+  const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+  const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);
+  const float2 wd = cmul(w, d);
+  return make_float2(e.x + 0.5f * wd.y, e.y - 0.5f * wd.x);
+}
+
+// Bin k in [0, M] of the untangled spectrum, computed exactly as the in-place
+// untangle pass does (so fused consumers see identical values).
+__device__ inline float2 untangled_bin(const float2* __restrict__ z, uint64_t M, uint64_t k, const RootTable& rt) {
+  if (k == 0) return untangle_one(z[0], z[0], root(rt, 0));
+  if (k == M) return untangle_one(z[0], z[0], make_float2(-1.f, 0.f));
+  if (2 * k <= M) return untangle_one(z[k], z[M - k], root(rt, k));
+  const float2 w = root(rt, M - k);
+  return untangle_one(z[k], z[M - k], make_float2(-w.x, w.y));
+}
+
 int plan_create_with(uint64_t n, int kind, uint32_t batch, const Decomp* forced, psx_fft_plan** out);
 int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t s);
+// Forward complex passes of an R2C plan whose first pass reads trial
+// blockIdx.z through the resampler (descs); writes the half-length complex
+// spectra Z (trial stride M) to zout without untangling.
+int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s);
+RootTable root_table(const psx_fft_plan* p);
+uint64_t complex_len(const psx_fft_plan* p);
+const Decomp& plan_decomp(const psx_fft_plan* p);
 
 }  // namespace fft
 }  // namespace psx
