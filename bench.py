@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""DM x acceleration trials/s of the acceleration-search core on one MI355X.
+
+Workload (one step): 64 dedispersed DM trials of 2^23 float32 samples at
+tsamp = 76 us - N(0,1) noise, 1/f^2 red noise (5x white amplitude at 0.1 Hz)
+and 6 injected accelerated pulsars - searched over the acceleration plan
+-100..+100 m/s^2 (tol 1.1, 64 us pulse width: 127 trials per DM, 8128 trials
+per step) with 4 harmonic folds and S/N threshold 7.  Series are generated
+on the device before timing, so inputs are resident in HBM.  A step is the
+whole search: the 64 per-DM prologues plus all 8128 trials through the
+batched, fused, graph-launched path, with host clustering and distilling.
+
+Also reported on the same inputs: the plain one-call-per-kernel Worker form
+on the GPU (one DM, all its trials), the CPU oracle on the host cores (a
+bounded sample of DM 0's trials), per-launch times of the fused kernels from
+HIP events on the batch stream, and the roofline of the dominant kernel.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0
+METRIC = "DM×accel trials/sec at 2^23 samples vs plain GPU; p50/p99 segment latency"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(n, npass):
+    """Algorithmic HBM bytes per trial of each fused launch: FFT pass 0 reads
+    the n-sample series through the resampler and writes n/2 complex; every
+    other FFT pass reads and writes n/2 complex; the spectrum pass reads n/2
+    complex and writes n/2+1 floats; the harmonic/threshold pass reads n/2+1
+    floats (its gathers hit the caches; candidates are negligible)."""
+    m, nb = n // 2, n // 2 + 1
+    b = [4 * n + 8 * m] + [16 * m] * (npass - 1)
+    b += [8 * m + 4 * nb, 4 * nb]
+    return b
+
+
+def profile_stages(P, W, args_c, opts, d_series, nsamps, accs, reps=10):
+    P.register_signatures({"psx_batch_profile": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
+                                                           C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_int,
+                                                           C.c_void_p, C.POINTER(C.c_int), C.c_char_p,
+                                                           C.c_size_t])})
+    accs = np.ascontiguousarray(accs, np.float32)
+    ms = np.zeros(8, np.float64)
+    ns = C.c_int(8)
+    desc = C.create_string_buffer(128)
+    P.check(P.lib().psx_batch_profile(C.byref(args_c), C.byref(opts), d_series, nsamps, accs.ctypes.data,
+                                      accs.size, reps, ms.ctypes.data, C.byref(ns), desc, 128))
+    return ms[:ns.value].tolist(), desc.value.decode()
+
+
+def cpu_baseline(O, W, w, series0, accs, threads, target_s=20.0):
+    """Oracle on the host cores: DM 0 prologue + a bounded sample of its trials."""
+    p = O.make_params(w.size, np.float32(w.tsamp), nharmonics=w.nharmonics, min_snr=w.min_snr)
+    t0 = time.perf_counter()
+    tim, mean, rms, std = O.prologue(series0, p)
+    t_pro = time.perf_counter() - t0
+    # one round of `threads` trials to estimate the rate, then size the sample
+    t0 = time.perf_counter()
+    O.search_accels(tim, p, accs[:threads], mean, std, nthreads=threads)
+    t_one = time.perf_counter() - t0
+    rounds = int(max(1, min(len(accs) // threads, (target_s - t_one - t_pro) // max(t_one, 1e-3))))
+    n = min(len(accs), rounds * threads)
+    t0 = time.perf_counter()
+    O.search_accels(tim, p, accs[:n], mean, std, nthreads=threads)
+    t_acc = time.perf_counter() - t0
+    # per-DM prologue amortised over the DM's full trial list
+    per_trial = t_acc / n + t_pro / len(accs)
+    return {"value": round(1.0 / per_trial, 3), "unit": "trials/s", "cores": threads, "kind": "port",
+            "sample": "DM 0 of the workload: prologue (%.1f s) + %d of its %d acceleration trials (%.1f s), "
+                      "prologue amortised over %d trials; double-precision CPU FFT" %
+                      (t_pro, n, len(accs), t_acc, len(accs))}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ndm", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=0, help="trials per fused batch (0 = tuned/default)")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (DM trials would shard across "
+              "GPUs with no collective on the hot path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import peasoup_amd as P
+    import peasoup_amd.workloads as W
+    from peasoup_amd import tuning
+
+    if not P.gpu_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    dev = P.device_info(0)
+    log("device", dev)
+    w = W.config("gpu64x2^23")
+    w.ndm = a.ndm
+    w.injections = [i for i in w.injections if i.trial < w.ndm]
+    args_c = w.search_args()
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    ntrials = w.ndm * accs.size
+    tuned = tuning.lookup(w.size, dev["name"])
+    batch = a.batch or tuned.get("batch", 16)
+    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"))
+    log("workload %s: %d DMs x %d accels = %d trials, batch %d, decomp %s" %
+        (w.name, w.ndm, accs.size, ntrials, batch, tuned.get("decomp")))
+
+    t0 = time.perf_counter()
+    d_series = W.generate_device(w)
+    P.synchronize()
+    log("generated %.1f GB of series in %.1f s" % (d_series.nbytes / 1e9, time.perf_counter() - t0))
+
+    def step():
+        return W.batch_search(args_c, d_series, w.dm_list(), accs, opts, nsamps=w.size)
+
+    first = None
+    for i in range(a.warmup):
+        c, s = step()
+        first = c
+        log("warmup %d: %d candidates, prologue %.3f s, trials %.3f s" % (i, c.size, s[0], s[1]))
+    P.synchronize()
+    t0 = time.perf_counter()
+    secs = []
+    for i in range(a.steps):
+        c, s = step()
+        secs.append(s.tolist())
+        if first is not None and not np.array_equal(c, first):
+            log("WARNING: step %d candidates differ from the warm-up's" % i)
+        first = c
+    P.synchronize()
+    dt = time.perf_counter() - t0
+    value = a.steps * ntrials / dt
+    log("timed %d steps: %.3f s -> %.1f trials/s" % (a.steps, dt, value))
+
+    # per-launch profile of one batch (HIP events on the batch stream)
+    stage_ms, desc = profile_stages(P, W, args_c, opts, d_series.ptr, w.size, accs[:batch], reps=10)
+    nb_trials = min(batch, accs.size)
+    npass = len(stage_ms) - 2
+    sbytes = [b * nb_trials for b in stage_bytes(w.size, npass)]
+    names = ["fft_pass0_resample"] + ["fft_pass%d" % q for q in range(1, npass)] + ["spec_post", "harm_peaks"]
+    stages = {nm: {"ms": round(ms, 4), "GBps": round(bb / (ms * 1e-3) / 1e9, 1), "bytes": bb}
+              for nm, ms, bb in zip(names, stage_ms, sbytes)}
+    dom = int(np.argmax(stage_ms))
+    achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = tuning.traffic(w.size, names[dom], nb_trials)
+    roofline = {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": sbytes[dom], "launch_trials": nb_trials, "fft_plan": desc}
+    log("stages", stages)
+
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": "trials/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d DM trials x 2^23 float32 samples, tsamp 76 us, N(0,1) + red noise + %d pulsars; "
+                               "accel plan -100..+100 m/s^2 tol 1.1 (127 trials/DM); 4 harmonic folds; S/N 7"
+                               % (w.ndm, len(w.injections)),
+                   "fft_size": w.size, "ndm": w.ndm, "trials_per_step": ntrials, "batch": batch,
+                   "fft_plan": desc, "graphs": not a.no_graph, "device": dev["name"]},
+        "roofline": roofline, "stages": stages,
+        "algorithmic_GBps": round(value * (12 * w.size + 68 * (w.size // 2 + 1)) / 1e9, 1),
+        "candidates": int(first.size) if first is not None else None,
+    }
+
+    series0 = None
+    if not a.no_plain:
+        series0 = d_series.numpy()[0] if w.ndm > 0 else None
+        c, s = W.worker_search(args_c, series0[None, :], np.zeros(1, np.float32), accs)
+        plain = accs.size / s[1]
+        result["plain_gpu"] = {"trials_per_s": round(plain, 2), "prologue_s": round(s[0], 4),
+                               "sample": "DM 0, all %d trials, one library call per reference call" % accs.size}
+        result["speedup_vs_plain"] = round(value / plain, 2)
+        log("plain GPU: %.1f trials/s" % plain)
+    if not a.no_cpu:
+        import oracle as O
+        if series0 is None:
+            series0 = d_series.numpy()[0]
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cb = cpu_baseline(O, W, w, series0, accs, threads, a.cpu_seconds)
+        result["cpu_baseline"] = cb
+        result["speedup_vs_cpu"] = round(value / cb["value"], 1)
+        log("cpu baseline", cb)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

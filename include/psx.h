@@ -147,6 +147,11 @@ int psx_fft_plan_destroy(psx_fft_plan* plan);
  * C2R: in = batch x (n/2+1) psx_c32, out = batch x n float (scaled by n).
  * in and out may not alias. */
 int psx_fft_exec(psx_fft_plan* plan, const void* in, void* out, psx_stream s);
+/* Host-only check (no device work): builds the pass geometry of a plan
+ * (npass = 0: default decomposition, else lens/cols per pass) and verifies
+ * that every launch reads and writes inside its buffers, for the plain
+ * transform and for the fused batched search. */
+int psx_fft_layout_check(uint64_t n, int kind, uint32_t batch, int npass, const int* lens, const int* cols);
 /* Number of kernel launches one exec enqueues (for profiling). */
 int psx_fft_plan_describe(const psx_fft_plan* plan, char* buf, size_t cap);
 
@@ -213,6 +218,87 @@ typedef struct psx_candidate {
 int psx_worker_search(const psx_search_args* args, const float* series, uint64_t nsamps, const float* dm_list,
                       uint32_t ndm, const float* accs, uint32_t naccs, psx_candidate* out, uint32_t cap,
                       uint32_t* count, double* seconds);
+
+/* Options of the batched search (psx_batch_search).  Zero fields take
+ * defaults; fft_npass > 0 forces an FFT decomposition (autotuner). */
+typedef struct psx_batch_opts {
+  uint32_t batch;          /* trials per fused batch (default 16) */
+  uint32_t cand_capacity;  /* above-threshold bins per batch (default 2^20) */
+  int32_t use_graph;       /* launch each batch as one captured hipGraph */
+  int32_t fft_npass;       /* 0 = default decomposition */
+  int32_t fft_len[3];      /* pass lengths (product = n/2) */
+  int32_t fft_cols[3];     /* column tile per pass */
+} psx_batch_opts;
+
+/* Same contract as psx_worker_search, computed by the batched path: all DM
+ * prologues, then every (DM, acceleration) trial through fused batches
+ * (resample+FFT, untangle+interbin+normalise, harmonic sums+threshold), with
+ * the Worker's per-trial HarmonicDistiller and per-DM AccelerationDistiller.
+ * series is a device pointer when series_on_device != 0.  seconds[0..2]
+ * (optional) = prologues, trials, total. */
+int psx_batch_search(const psx_search_args* args, const psx_batch_opts* opts, const float* series,
+                     int series_on_device, uint64_t nsamps, const float* dm_list, uint32_t ndm, const float* accs,
+                     uint32_t naccs, psx_candidate* out, uint32_t cap, uint32_t* count, double* seconds);
+
+/* Times each launch of one fused batch (min(naccs, batch) trials over the
+ * first series of d_series, after its prologue) with HIP events on the batch
+ * stream, averaged over reps.  stage_ms[0..*nstage) = the FFT passes (the
+ * first with the fused resampling), untangle+interbin+normalise, harmonic
+ * sums+threshold; desc receives the FFT decomposition. */
+int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
+                      uint64_t nsamps, const float* accs, uint32_t naccs, int reps, double* stage_ms, int* nstage,
+                      char* desc, size_t cap);
+
+/* ------------------------------------------------------------------ */
+/* Serving: continuous batching of segment-search requests             */
+/* ------------------------------------------------------------------ */
+typedef struct psx_server psx_server;
+typedef struct psx_server_config {
+  uint64_t hbm_budget_bytes;  /* resident series + engine buffers never exceed this */
+  uint32_t batch;             /* trials per fused batch (default 16) */
+  uint32_t max_active;        /* admitted requests at once (0 = budget only) */
+  int32_t use_graph;          /* graph-launched batches */
+} psx_server_config;
+
+/* One beam: ndm dedispersed series of nsamps samples (host or device memory,
+ * valid until the request completes) searched with args. */
+typedef struct psx_request {
+  psx_search_args args;
+  const float* series;
+  int32_t series_on_device;
+  uint64_t nsamps;
+  const float* dm_list;
+  uint32_t ndm;
+  const float* accs;  /* optional acceleration list (else AccelerationPlan) */
+  uint32_t naccs;
+} psx_request;
+
+enum { PSX_REQ_WAITING = 0, PSX_REQ_ACTIVE = 1, PSX_REQ_DONE = 2, PSX_REQ_FAILED = 3 };
+typedef struct psx_request_status {
+  int32_t state;
+  int32_t error;
+  uint32_t ncand;
+  uint64_t ntrials;
+  double t_submit, t_admit, t_done; /* seconds on the library's monotonic clock */
+} psx_request_status;
+
+typedef struct psx_server_stats {
+  uint64_t used_bytes, budget_bytes;
+  uint32_t waiting, active, engines;
+  uint64_t completed, failed, trials_done;
+} psx_server_counters;
+
+int psx_server_create(const psx_server_config* cfg, psx_server** server);
+/* Non-blocking.  PSX_ERR_BUDGET when the request alone cannot fit. */
+int psx_server_submit(psx_server* server, const psx_request* req, uint64_t* id);
+int psx_server_poll(psx_server* server, uint64_t id, psx_request_status* status);
+int psx_server_wait(psx_server* server, uint64_t id, double timeout_s, psx_request_status* status);
+/* Candidates of a completed request (per DM, accel-distilled, DM order);
+ * releases the request. */
+int psx_server_result(psx_server* server, uint64_t id, psx_candidate* out, uint32_t cap, uint32_t* count);
+int psx_server_stats(psx_server* server, psx_server_counters* stats);
+/* Completes queued and running requests, then frees everything. */
+int psx_server_destroy(psx_server* server);
 
 #ifdef __cplusplus
 }

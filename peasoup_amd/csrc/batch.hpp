@@ -1,0 +1,38 @@
+// batch.hpp - device entry points of the fused batched search.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fft_plan.hpp"
+
+namespace psx {
+namespace batch {
+
+// One above-threshold bin: trial slot, harmonic number (0 = fundamental),
+// bin index and S/N.
+struct Cand16 {
+  uint32_t slot;
+  uint32_t nh;
+  int32_t idx;
+  float snr;
+};
+
+// PeakFinder windows per harmonic number nh = 0..F (start inclusive, end
+// exclusive) and the bin range [lo, hi) covering all of them.
+struct Windows {
+  int32_t start[6];
+  int32_t end[6];
+  int64_t lo, hi;
+  float thr;
+};
+
+int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, const psx::fft::RootTable& rt,
+                     const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s);
+int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_t nfolds,
+                      const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count,
+                      hipStream_t s);
+int launch_zero(uint32_t* p, hipStream_t s);
+
+}  // namespace batch
+}  // namespace psx

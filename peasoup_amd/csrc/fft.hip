@@ -391,7 +391,7 @@ struct psx_fft_plan {
 
 namespace {
 
-int build_passes(psx_fft_plan* p) {
+int build_passes(psx_fft_plan* p, bool device_tables) {
   using psx::fft::Decomp;
   const Decomp& d = p->d;
   const uint64_t M = p->M;
@@ -430,8 +430,10 @@ int build_passes(psx_fft_plan* p) {
   while ((1ull << (rsh + 1)) * (1ull << (rsh + 1)) <= n && (n % (1ull << (rsh + 1))) == 0) ++rsh;
   root_table_host(n, rsh, lo, hi);
   const size_t rlo = append(lo), rhi = append(hi);
-  PSX_CHECK_HIP(hipMalloc(&p->tables, host.size() * sizeof(float2)));
-  PSX_CHECK_HIP(hipMemcpy(p->tables, host.data(), host.size() * sizeof(float2), hipMemcpyHostToDevice));
+  if (device_tables) {
+    PSX_CHECK_HIP(hipMalloc(&p->tables, host.size() * sizeof(float2)));
+    PSX_CHECK_HIP(hipMemcpy(p->tables, host.data(), host.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
   p->rt = RootTable{p->tables + rlo, p->tables + rhi, rsh, (uint32_t)((1ull << rsh) - 1)};
 
   // --- passes
@@ -539,7 +541,7 @@ int plan_create_with(uint64_t n, int kind, uint32_t batch, const Decomp* forced,
     delete p;
     return psx::fail(PSX_ERR_ARG, "psx_fft_plan_create: decomposition does not multiply to M");
   }
-  rc = build_passes(p);
+  rc = build_passes(p, true);
   if (!rc) {
     const bool need_work = (kind == PSX_FFT_C2R) || p->d.npass > 1;
     if (need_work) {
@@ -555,6 +557,9 @@ int plan_create_with(uint64_t n, int kind, uint32_t batch, const Decomp* forced,
   return psx::ok();
 }
 
+int check_pass_bounds(const PassLaunch& L, const PassArgs& a, uint32_t gz, uint64_t cap_in, uint64_t cap_out,
+                      uint64_t m_lim);
+
 int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t hs) {
   const unsigned tb = 256;
   if (p->kind == PSX_FFT_C2R) {
@@ -567,6 +572,12 @@ int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t hs) {
     PassArgs a = L.a;
     a.in = L.src == 0 ? (const float2*)in : (L.src == 1 ? p->work : (const float2*)out);
     a.out = L.dst == 2 ? (float2*)out : p->work;
+    // capacities in complex elements: user input (R2C) n/2 per transform,
+    // work n/2, user output n/2+1 (R2C) or n/2 (C2R) per transform
+    const uint64_t cap_in = p->M * p->batch;
+    const uint64_t cap_out = L.dst == 2 ? (p->kind == PSX_FFT_R2C ? p->nb : p->M) * p->batch : p->M * p->batch;
+    int rc = check_pass_bounds(L, a, L.gz, cap_in, cap_out, 0);
+    if (rc) return rc;
     hipLaunchKernelGGL(L.k->fn, dim3(L.gx, L.gy, L.gz), dim3(L.k->threads), L.k->lds, hs, a);
     PSX_CHECK_LAUNCH("k_fft_pass");
   }
@@ -578,24 +589,63 @@ int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t hs) {
   return PSX_OK;
 }
 
+// Largest element index a pass launch reads and writes (all strides are
+// non-negative), checked against the capacities of its buffers before the
+// launch.  cap_in == 0 skips the input check; m_lim > 0 checks the complex
+// index of a resampling first pass (m < m_lim keeps 2m+1 < n).
+int check_pass_bounds(const PassLaunch& L, const PassArgs& a, uint32_t gz, uint64_t cap_in, uint64_t cap_out,
+                      uint64_t m_lim) {
+  const uint64_t C = (uint64_t)L.k->C, Ln = (uint64_t)L.k->L;
+  const uint64_t cols = (uint64_t)L.gx * C - 1, gy = (uint64_t)L.gy - 1, gzz = (uint64_t)gz - 1;
+  if (a.in_col < 0 || a.in_elem < 0 || a.in_sub < 0 || a.in_tr < 0 || a.out_col < 0 || a.out_elem < 0 ||
+      a.out_sub < 0 || a.out_tr < 0)
+    return psx::fail(PSX_ERR_ARG, "FFT pass: negative stride");
+  const uint64_t in_max = gzz * a.in_tr + gy * a.in_sub + cols * a.in_col + (Ln - 1) * a.in_elem;
+  const uint64_t out_max = gzz * a.out_tr + gy * a.out_sub + cols * a.out_col + (Ln - 1) * a.out_elem;
+  const uint64_t m_max = gy * a.in_sub + cols * a.in_col + (Ln - 1) * a.in_elem;
+  if (cap_in && in_max >= cap_in)
+    return psx::fail(PSX_ERR_ARG, "FFT pass (L=%d,C=%d) would read element %llu of %llu", L.k->L, L.k->C,
+                     (unsigned long long)in_max, (unsigned long long)cap_in);
+  if (out_max >= cap_out)
+    return psx::fail(PSX_ERR_ARG, "FFT pass (L=%d,C=%d) would write element %llu of %llu", L.k->L, L.k->C,
+                     (unsigned long long)out_max, (unsigned long long)cap_out);
+  if (m_lim && m_max >= m_lim)
+    return psx::fail(PSX_ERR_ARG, "FFT pass (L=%d,C=%d) would resample beyond the series", L.k->L, L.k->C);
+  return PSX_OK;
+}
+
+int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs) {
+  if (p->kind != PSX_FFT_R2C || ntrials < 1 || ntrials > p->batch || q < 0 || q >= (int)p->passes.size())
+    return psx::fail(PSX_ERR_ARG, "exec_fused_pass: bad plan, pass or trial count");
+  PassLaunch& L = p->passes[q];
+  PassArgs a = L.a;
+  a.in = L.src == 1 ? p->work : nullptr;
+  a.out = L.dst == 2 ? zout : p->work;
+  // zout holds the half-length spectra Z of the trials back to back (stride
+  // M), not the M+1-bin R2C output layout of the plain plan.
+  if (L.dst == 2) a.out_tr = (int64_t)p->M;
+  if (q == 0) {
+    a.resample = 1;
+    a.rs_n = p->n;
+    a.descs = d_descs;
+  }
+  const uint64_t cap_in = p->M * p->batch, cap_out = L.dst == 2 ? p->M * p->batch : p->M * p->batch;
+  int rc = check_pass_bounds(L, a, ntrials, q == 0 ? 0 : cap_in, cap_out, q == 0 ? p->M : 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(L.k->fn, dim3(L.gx, L.gy, ntrials), dim3(L.k->threads), L.k->lds, hs, a);
+  PSX_CHECK_LAUNCH("k_fft_pass(fused)");
+  return PSX_OK;
+}
+
 int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs) {
-  if (p->kind != PSX_FFT_R2C || ntrials < 1 || ntrials > p->batch)
-    return psx::fail(PSX_ERR_ARG, "exec_fused_r2c: bad plan or trial count");
-  for (size_t q = 0; q < p->passes.size(); ++q) {
-    PassLaunch& L = p->passes[q];
-    PassArgs a = L.a;
-    a.in = L.src == 1 ? p->work : nullptr;
-    a.out = L.dst == 2 ? zout : p->work;
-    if (q == 0) {
-      a.resample = 1;
-      a.rs_n = p->n;
-      a.descs = d_descs;
-    }
-    hipLaunchKernelGGL(L.k->fn, dim3(L.gx, L.gy, ntrials), dim3(L.k->threads), L.k->lds, hs, a);
-    PSX_CHECK_LAUNCH("k_fft_pass(fused)");
+  for (int q = 0; q < (int)p->passes.size(); ++q) {
+    int rc = exec_fused_pass(p, q, d_descs, ntrials, zout, hs);
+    if (rc) return rc;
   }
   return PSX_OK;
 }
+
+int num_passes(const psx_fft_plan* p) { return (int)p->passes.size(); }
 
 RootTable root_table(const psx_fft_plan* p) { return p->rt; }
 uint64_t complex_len(const psx_fft_plan* p) { return p->M; }
@@ -634,5 +684,52 @@ extern "C" int psx_fft_plan_describe(const psx_fft_plan* p, char* buf, size_t ca
   s += " launches=" + std::to_string(p->passes.size() + 1);
   strncpy(buf, s.c_str(), cap - 1);
   buf[cap - 1] = 0;
+  return psx::ok();
+}
+
+// Host-only geometry check of a plan: builds the pass launches without device
+// tables and validates every pass's largest read and write index against its
+// buffers, for the plain transform and for the fused batched use.
+extern "C" int psx_fft_layout_check(uint64_t n, int kind, uint32_t batch, int npass, const int* lens, const int* cols) {
+  PSX_REQUIRE(n >= 16 && (n % 2) == 0 && psx::fft::valid_complex_len(n / 2), "psx_fft_layout_check: bad length");
+  PSX_REQUIRE(kind == PSX_FFT_R2C || kind == PSX_FFT_C2R, "psx_fft_layout_check: bad kind");
+  PSX_REQUIRE(batch >= 1 && batch <= 65535, "psx_fft_layout_check: bad batch");
+  psx_fft_plan p;
+  p.n = n;
+  p.M = n / 2;
+  p.nb = n / 2 + 1;
+  p.kind = kind;
+  p.batch = batch;
+  bool okd;
+  if (npass > 0) {
+    PSX_REQUIRE(npass <= 3 && lens && cols, "psx_fft_layout_check: bad decomposition");
+    p.d.npass = npass;
+    for (int q = 0; q < npass; ++q) {
+      p.d.L[q] = lens[q];
+      p.d.C[q] = cols[q];
+    }
+    okd = psx::fft::pick_tiles(p.d);
+  } else {
+    okd = psx::fft::default_decomp(p.M, p.d);
+  }
+  if (!okd || !psx::fft::decomp_supported(p.d)) return psx::fail(PSX_ERR_UNSUPPORTED, "no decomposition");
+  uint64_t prod = 1;
+  for (int q = 0; q < p.d.npass; ++q) prod *= (uint64_t)p.d.L[q];
+  PSX_REQUIRE(prod == p.M, "psx_fft_layout_check: decomposition does not multiply to n/2");
+  int rc = build_passes(&p, false);
+  if (rc) return rc;
+  for (size_t q = 0; q < p.passes.size(); ++q) {
+    const PassLaunch& L = p.passes[q];
+    const uint64_t cap_in = p.M * p.batch;
+    const uint64_t cap_out = L.dst == 2 ? (kind == PSX_FFT_R2C ? p.nb : p.M) * p.batch : p.M * p.batch;
+    rc = psx::fft::check_pass_bounds(L, L.a, L.gz, cap_in, cap_out, 0);
+    if (rc) return rc;
+    if (kind == PSX_FFT_R2C) {
+      PassArgs a = L.a;
+      if (L.dst == 2) a.out_tr = (int64_t)p.M;
+      rc = psx::fft::check_pass_bounds(L, a, p.batch, q == 0 ? 0 : cap_in, p.M * p.batch, q == 0 ? p.M : 0);
+      if (rc) return rc;
+    }
+  }
   return psx::ok();
 }

@@ -57,6 +57,8 @@ int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t s);
 // blockIdx.z through the resampler (descs); writes the half-length complex
 // spectra Z (trial stride M) to zout without untangling.
 int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s);
+int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s);
+int num_passes(const psx_fft_plan* p);
 RootTable root_table(const psx_fft_plan* p);
 uint64_t complex_len(const psx_fft_plan* p);
 const Decomp& plan_decomp(const psx_fft_plan* p);

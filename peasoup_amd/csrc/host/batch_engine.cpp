@@ -1,0 +1,332 @@
+// batch_engine.cpp - host side of the batched acceleration search.
+#include "batch_engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "common.hpp"
+#include "kernels_dev.hpp"
+#include "peasoup/distiller.hpp"
+#include "search_common.hpp"
+
+namespace psx {
+namespace batch {
+
+namespace {
+constexpr uint32_t kPrefix = 8192;  // candidates copied back with every batch
+
+void hip_ok(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+void psx_ok(int rc, const char* what) {
+  if (rc != PSX_OK) throw std::runtime_error(std::string(what) + ": " + psx_last_error_string());
+}
+template <class T>
+void dalloc(T** p, uint64_t n) {
+  hip_ok(hipMalloc((void**)p, sizeof(T) * (n ? n : 1)), "hipMalloc");
+}
+template <class T>
+void halloc(T** p, uint64_t n) {
+  hip_ok(hipHostMalloc((void**)p, sizeof(T) * (n ? n : 1), hipHostMallocDefault), "hipHostMalloc");
+}
+int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+}  // namespace
+
+uint64_t Engine::device_bytes(uint64_t N, uint32_t B, uint32_t cap) {
+  const uint64_t M = N / 2, nb = M + 1;
+  const uint64_t pst = (uint64_t)round_up((int64_t)nb, 64);
+  uint64_t b = 0;
+  b += 2 * M * B * sizeof(float2);      // FFT work + Z
+  b += pst * B * sizeof(float);         // P
+  b += 2 * (uint64_t)cap * sizeof(Cand16);
+  b += nb * (sizeof(float2) + 2 * sizeof(float)) + nb / 4 * sizeof(float);  // prologue X, A, median
+  b += 2 * M * sizeof(float2) + 16 * 1024 * 1024;                          // prologue plans + tables
+  return b;
+}
+
+Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t stream)
+    : args(a), N(a.size), M(a.size / 2), nb(a.size / 2 + 1), B(opt.batch), cap(opt.cand_capacity),
+      F(a.nharmonics), use_graph(opt.use_graph), s(stream) {
+  if (B < 1 || B > 1024) throw std::runtime_error("batch must be 1..1024");
+  if (cap < kPrefix) cap = kPrefix;
+  pstride = round_up((int64_t)nb, 64);
+  psx_ok(psx::fft::plan_create_with(N, PSX_FFT_R2C, B, opt.decomp, &plan), "batched R2C plan");
+  rt = psx::fft::root_table(plan);
+  dalloc(&Z, M * B);
+  dalloc(&P, (uint64_t)pstride * B);
+  for (Slot& sl : slots) {
+    dalloc(&sl.d_desc, B);
+    halloc(&sl.h_desc, B);
+    dalloc(&sl.d_cands, cap);
+    halloc(&sl.h_cands, kPrefix);
+    dalloc(&sl.d_count, 1);
+    halloc(&sl.h_count, 1);
+    hip_ok(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+  }
+  // PeakFinder windows and bin->frequency factors per harmonic number
+  // (pipeline_multi.cu:130-131: float tobs = size*tsamp; bin_width = 1.0/tobs)
+  const float tobs = (unsigned int)N * args.tsamp;
+  const float bin_width = 1.0 / tobs;
+  win.lo = INT64_MAX;
+  win.hi = 0;
+  for (uint32_t nh = 0; nh <= F; ++nh) {
+    const peasoup::PeakWindow w = peasoup::peak_window((unsigned)nb, (double)bin_width, nh, args.min_freq,
+                                                       args.max_freq);
+    win.start[nh] = std::max(w.start, 0);
+    win.end[nh] = std::min(w.end, (int)nb);
+    factor[nh] = w.factor;
+    if (win.end[nh] > win.start[nh]) {
+      win.lo = std::min<int64_t>(win.lo, win.start[nh]);
+      win.hi = std::max<int64_t>(win.hi, win.end[nh]);
+    }
+  }
+  if (win.hi <= win.lo) win.lo = win.hi = 0;
+  win.thr = args.min_snr;
+  // prologue resources (plain path)
+  psx_ok(psx_fft_plan_create(N, PSX_FFT_R2C, 1, &p_r2c), "prologue R2C plan");
+  psx_ok(psx_fft_plan_create(N, PSX_FFT_C2R, 1, &p_c2r), "prologue C2R plan");
+  dalloc(&X, nb);
+  dalloc(&A, nb);
+  dalloc(&med, nb);
+  dalloc(&med_work, psx_deredden_work_floats(nb));
+  dalloc((uint8_t**)&red_work, psx_reduce_work_bytes());
+}
+
+Engine::~Engine() {
+  if (s) (void)hipStreamSynchronize(s);
+  for (Slot& sl : slots) {
+    for (auto& g : sl.graphs) (void)hipGraphExecDestroy(g.second);
+    (void)hipFree(sl.d_desc);
+    (void)hipHostFree(sl.h_desc);
+    (void)hipFree(sl.d_cands);
+    (void)hipHostFree(sl.h_cands);
+    (void)hipFree(sl.d_count);
+    (void)hipHostFree(sl.h_count);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  (void)hipFree(Z);
+  (void)hipFree(P);
+  psx_fft_plan_destroy(plan);
+  psx_fft_plan_destroy(p_r2c);
+  psx_fft_plan_destroy(p_c2r);
+  (void)hipFree(X);
+  (void)hipFree(A);
+  (void)hipFree(med);
+  (void)hipFree(med_work);
+  (void)hipFree(red_work);
+}
+
+void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std) {
+  // pipeline_multi.cu:170-238
+  const uint64_t ncopy = std::min<uint64_t>(n_real, N);
+  if (d_series != d_tim) hip_ok(hipMemcpyAsync(d_tim, d_series, ncopy * sizeof(float), hipMemcpyDeviceToDevice, s),
+                                "prologue copy");
+  psx_ok(psx_remove_baseline(d_tim, ncopy, red_work, s), "remove_baseline");
+  if (N > n_real) psx_ok(psx_fill(d_tim, n_real, N, 0.0f, s), "fill");
+  const float tobs = (unsigned int)N * args.tsamp;
+  const float bin_width = 1.0 / tobs;
+  psx_ok(psx_fft_exec(p_r2c, d_tim, X, s), "prologue R2C");
+  psx_ok(psx_form_amplitude(X, A, nb, s), "form");
+  psx_ok(psx_deredden_median(A, nb, (double)bin_width, 0.05f, 0.5f, med, med_work, s), "calculate_median");
+  psx_ok(psx_divide_c_by_f(X, med, nb, s), "deredden");
+  psx_ok(psx_form_interbin(X, A, nb, s), "form_interpolated");
+  float m = 0, r = 0;
+  psx_ok(psx_mean_rms(A, nb, 0, &m, &r, red_work, s), "stats");
+  *mean = m;
+  *std = std::sqrt(r * r - m * m);
+  psx_ok(psx_fft_exec(p_c2r, X, d_tim, s), "prologue C2R");
+}
+
+void Engine::enqueue(Slot& sl, uint32_t n) {
+  auto record = [&]() {
+    hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s),
+           "descriptor upload");
+    psx_ok(launch_zero(sl.d_count, s), "zero");
+    psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s), "fused FFT");
+    psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+    psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
+    hip_ok(hipMemcpyAsync(sl.h_count, sl.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count copy");
+    hip_ok(hipMemcpyAsync(sl.h_cands, sl.d_cands, kPrefix * sizeof(Cand16), hipMemcpyDeviceToHost, s),
+           "candidate copy");
+  };
+  if (use_graph) {
+    auto it = sl.graphs.find(n);
+    if (it == sl.graphs.end()) {
+      hipGraph_t g;
+      hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+      try {
+        record();
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        throw;
+      }
+      hip_ok(hipStreamEndCapture(s, &g), "end capture");
+      hipGraphExec_t ge;
+      hip_ok(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
+      (void)hipGraphDestroy(g);
+      it = sl.graphs.emplace(n, ge).first;
+    }
+    hip_ok(hipGraphLaunch(it->second, s), "graph launch");
+  } else {
+    record();
+  }
+  hip_ok(hipEventRecord(sl.done, s), "event record");
+}
+
+void Engine::submit(const TrialWork* trials, uint32_t n) {
+  if (n == 0) return;
+  if (n > B) throw std::runtime_error("submit: more trials than the batch size");
+  if (pending.size() == 2) {
+    ready.emplace_back();
+    collect_oldest(ready.back());
+  }
+  Slot& sl = slots[next_slot];
+  sl.meta.assign(trials, trials + n);
+  for (uint32_t t = 0; t < n; ++t) {
+    const TrialWork& w = trials[t];
+    psx::fft::TrialDesc d;
+    d.tim = w.tim;
+    d.af = psx::accel_fact(w.acc, args.tsamp);
+    if (!(fabs(d.af) * (double)N < 1.0)) throw std::runtime_error("acceleration too large for the series length");
+    d.mean_n = w.mean * (unsigned int)N;  // stats::normalise(pspec, mean*size, std*size, ...)
+    d.std_n = w.std * (unsigned int)N;
+    d.slot = t;
+    d.pad = 0;
+    sl.h_desc[t] = d;
+  }
+  enqueue(sl, n);
+  pending.push_back(next_slot);
+  next_slot ^= 1;
+}
+
+bool Engine::collect(std::vector<std::vector<Candidate>>& per_trial) {
+  if (!ready.empty()) {
+    per_trial = std::move(ready.front());
+    ready.erase(ready.begin());
+    return true;
+  }
+  return collect_oldest(per_trial);
+}
+
+bool Engine::ready_or_done() {
+  if (!ready.empty()) return true;
+  if (pending.empty()) return false;
+  const hipError_t q = hipEventQuery(slots[pending.front()].done);
+  if (q == hipErrorNotReady) return false;
+  hip_ok(q, "batch query");
+  return true;
+}
+
+bool Engine::collect_oldest(std::vector<std::vector<Candidate>>& per_trial) {
+  if (pending.empty()) return false;
+  Slot& sl = slots[pending.front()];
+  pending.erase(pending.begin());
+  hip_ok(hipEventSynchronize(sl.done), "batch wait");
+  process(sl, per_trial);
+  return true;
+}
+
+void Engine::process(Slot& sl, std::vector<std::vector<Candidate>>& per_trial) {
+  const uint32_t n = (uint32_t)sl.meta.size();
+  const uint32_t count = *sl.h_count;
+  if (count > cap)
+    throw std::runtime_error("batched search: " + std::to_string(count) + " bins above threshold exceed the " +
+                             std::to_string(cap) + "-entry candidate buffer");
+  std::vector<Cand16> all(sl.h_cands, sl.h_cands + std::min(count, kPrefix));
+  if (count > kPrefix) {
+    all.resize(count);
+    hip_ok(hipMemcpy(all.data() + kPrefix, sl.d_cands + kPrefix, (count - kPrefix) * sizeof(Cand16),
+                     hipMemcpyDeviceToHost),
+           "candidate tail copy");
+  }
+  // order by (trial, harmonic, bin): the stable order device_find_peaks
+  // produces for each spectrum
+  std::sort(all.begin(), all.end(), [](const Cand16& a, const Cand16& b) {
+    if (a.slot != b.slot) return a.slot < b.slot;
+    if (a.nh != b.nh) return a.nh < b.nh;
+    return a.idx < b.idx;
+  });
+  per_trial.assign(n, {});
+  HarmonicDistiller harm(args.freq_tol, args.max_harm, false);
+  std::vector<int> idxs, pidx;
+  std::vector<float> snrs, psnr, pfreq;
+  size_t pos = 0;
+  for (uint32_t t = 0; t < n; ++t) {
+    const TrialWork& w = sl.meta[t];
+    SpectrumCandidates sc(w.dm, w.dm_idx, w.acc);
+    for (uint32_t nh = 0; nh <= F; ++nh) {
+      idxs.clear();
+      snrs.clear();
+      while (pos < all.size() && all[pos].slot == t && all[pos].nh == nh) {
+        idxs.push_back(all[pos].idx);
+        snrs.push_back(all[pos].snr);
+        ++pos;
+      }
+      // PeakFinder host buffers hold 100000 entries (peakfinder.hpp:61-67)
+      if (idxs.size() > 100000) throw std::runtime_error("more than 100000 bins above threshold in one spectrum");
+      pidx.resize(idxs.size() + 1);
+      psnr.resize(idxs.size() + 1);
+      const int np = peasoup::identify_unique_peaks(idxs.data(), snrs.data(), (int)idxs.size(), args.min_gap,
+                                                    pidx.data(), psnr.data());
+      pfreq.resize(np + 1);
+      for (int i = 0; i < np; ++i) pfreq[i] = pidx[i] * factor[nh];
+      sc.append(psnr.data(), pfreq.data(), (int)nh, np);
+    }
+    per_trial[t] = harm.distill(sc.cands);
+  }
+}
+
+void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& out) {
+  if (n < 1 || n > B) throw std::runtime_error("profile: 1..B trials");
+  if (!pending.empty() || !ready.empty()) throw std::runtime_error("profile: engine busy");
+  Slot& sl = slots[0];
+  for (uint32_t t = 0; t < n; ++t) {
+    psx::fft::TrialDesc d;
+    d.tim = trials[t].tim;
+    d.af = psx::accel_fact(trials[t].acc, args.tsamp);
+    d.mean_n = trials[t].mean * (unsigned int)N;
+    d.std_n = trials[t].std * (unsigned int)N;
+    d.slot = t;
+    d.pad = 0;
+    sl.h_desc[t] = d;
+  }
+  hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s), "desc");
+  const int np = psx::fft::num_passes(plan);
+  const int nstage = np + 2;
+  std::vector<hipEvent_t> ev(nstage + 1);
+  for (auto& e : ev) hip_ok(hipEventCreate(&e), "event");
+  std::vector<double> acc(nstage, 0.0);
+  for (int r = 0; r < reps + 1; ++r) {
+    psx_ok(launch_zero(sl.d_count, s), "zero");
+    hip_ok(hipEventRecord(ev[0], s), "event");
+    for (int q = 0; q < np; ++q) {
+      psx_ok(psx::fft::exec_fused_pass(plan, q, sl.d_desc, n, Z, s), "fused pass");
+      hip_ok(hipEventRecord(ev[q + 1], s), "event");
+    }
+    psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+    hip_ok(hipEventRecord(ev[np + 1], s), "event");
+    psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
+    hip_ok(hipEventRecord(ev[np + 2], s), "event");
+    hip_ok(hipEventSynchronize(ev[np + 2]), "event sync");
+    if (r == 0) continue;  // warm-up repetition
+    for (int k = 0; k < nstage; ++k) {
+      float ms = 0;
+      hip_ok(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "elapsed");
+      acc[k] += ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  out.ms.assign(nstage, 0.0);
+  for (int k = 0; k < nstage; ++k) out.ms[k] = acc[k] / reps;
+  out.npass = np;
+  const psx::fft::Decomp& d = psx::fft::plan_decomp(plan);
+  out.desc.clear();
+  for (int q = 0; q < d.npass; ++q)
+    out.desc += (q ? "x" : "") + std::to_string(d.L[q]) + "/C" + std::to_string(d.C[q]);
+}
+
+}  // namespace batch
+}  // namespace psx

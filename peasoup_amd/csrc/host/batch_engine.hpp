@@ -1,0 +1,124 @@
+// batch_engine.hpp - batched acceleration search for one FFT length.
+//
+// The engine owns the device buffers of one batch of up to B trials and two
+// sets of candidate/descriptor buffers, so the host can cluster and distill
+// the candidates of batch i (PeakFinder::identify_unique_peaks,
+// HarmonicDistiller - host code, semantics unchanged) while batch i+1 runs on
+// the device.  Each batch is one hipGraph launch (captured once per batch
+// shape) when graphs are enabled.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "fft_plan.hpp"
+#include "peasoup/candidates.hpp"
+#include "peasoup/transforms.hpp"
+#include "psx.h"
+
+namespace psx {
+namespace batch {
+
+struct TrialWork {
+  const float* tim;  // dereddened, inverse-transformed series of the DM (device)
+  float acc;
+  float mean;  // interbinned-spectrum stats of the DM (stats::stats)
+  float std;
+  float dm;
+  int dm_idx;
+};
+
+struct EngineOptions {
+  uint32_t batch = 16;
+  uint32_t cand_capacity = 1u << 20;
+  bool use_graph = true;
+  const psx::fft::Decomp* decomp = nullptr;  // FFT decomposition override (autotuner)
+};
+
+struct StageTimes {
+  std::vector<double> ms;  // per launch: FFT passes (first one with resampling), spec_post, harm_peaks
+  int npass = 0;
+  std::string desc;  // FFT decomposition
+};
+
+class Engine {
+ public:
+  Engine(const psx_search_args& args, const EngineOptions& opt, hipStream_t stream);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  uint32_t batch() const { return B; }
+  uint64_t size() const { return N; }
+  hipStream_t stream() const { return s; }
+  // Enqueue up to B trials.  At most two batches are in flight; submit()
+  // collects the oldest first when both slots are busy (its results are
+  // queued for the next collect()).
+  void submit(const TrialWork* trials, uint32_t n);
+  // Waits for the oldest in-flight batch and returns its per-trial
+  // harmonic-distilled candidate lists (trial order of the submit).
+  bool collect(std::vector<std::vector<Candidate>>& per_trial);
+  uint32_t in_flight() const { return (uint32_t)(pending.size() + ready.size()); }
+  // True when collect() would return without waiting on the device.
+  bool ready_or_done();
+
+  // Worker prologue (pipeline_multi.cu:170-238) for one DM with the plain
+  // kernels: d_series (n_real samples) -> d_tim (N samples), mean/std.
+  void prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std);
+
+  // Average duration of each launch of one batch of n trials, timed with HIP
+  // events on the engine stream over `reps` repetitions (after one warm-up).
+  void profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& out);
+
+  // Device bytes held by an engine of this shape (for admission control).
+  static uint64_t device_bytes(uint64_t N, uint32_t B, uint32_t cap);
+
+ private:
+  struct Slot {
+    psx::fft::TrialDesc* d_desc = nullptr;
+    psx::fft::TrialDesc* h_desc = nullptr;
+    Cand16* d_cands = nullptr;
+    Cand16* h_cands = nullptr;
+    uint32_t* d_count = nullptr;
+    uint32_t* h_count = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<TrialWork> meta;
+    std::map<uint32_t, hipGraphExec_t> graphs;
+  };
+  void enqueue(Slot& sl, uint32_t n);
+  bool collect_oldest(std::vector<std::vector<Candidate>>& per_trial);
+  void process(Slot& sl, std::vector<std::vector<Candidate>>& per_trial);
+
+  psx_search_args args;
+  uint64_t N, M, nb;
+  int64_t pstride;
+  uint32_t B, cap, F;
+  bool use_graph;
+  hipStream_t s;
+  psx_fft_plan* plan = nullptr;
+  psx::fft::RootTable rt{};
+  float2* Z = nullptr;
+  float* P = nullptr;
+  Windows win{};
+  float factor[6];
+  Slot slots[2];
+  std::vector<int> pending;  // slot indices in submission order
+  int next_slot = 0;
+  std::vector<std::vector<std::vector<Candidate>>> ready;  // collected early by submit()
+
+  // prologue resources
+  psx_fft_plan* p_r2c = nullptr;
+  psx_fft_plan* p_c2r = nullptr;
+  psx_c32* X = nullptr;
+  float* A = nullptr;
+  float* med = nullptr;
+  float* med_work = nullptr;
+  void* red_work = nullptr;
+};
+
+}  // namespace batch
+}  // namespace psx

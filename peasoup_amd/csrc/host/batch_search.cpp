@@ -1,0 +1,204 @@
+// batch_search.cpp - whole-search driver over the batched engine: the
+// Worker's DM loop (src/pipeline_multi.cu:155-295) restructured so that all
+// DM prologues run first and the acceleration trials of every DM stream
+// through fused, graph-launched batches.  Candidate semantics are the
+// Worker's: per-trial HarmonicDistiller, per-DM AccelerationDistiller over
+// the trials in acceleration-list order.
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <vector>
+
+#include "batch_engine.hpp"
+#include "common.hpp"
+#include "peasoup/distiller.hpp"
+#include "search_common.hpp"
+
+using psx::batch::Engine;
+using psx::batch::EngineOptions;
+using psx::batch::TrialWork;
+
+namespace {
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+}  // namespace
+
+namespace psx {
+namespace search {
+
+// Runs the accel trials of the given DMs (device series already prologued
+// into tims) and returns the accel-distilled candidates per DM.
+void run_trials(Engine& eng, const psx_search_args& a, const std::vector<const float*>& tims,
+                const std::vector<float>& means, const std::vector<float>& stds, const std::vector<float>& dms,
+                const std::vector<int>& dm_idx, const std::vector<std::vector<float>>& accs,
+                std::vector<std::vector<Candidate>>& per_dm) {
+  const size_t ndm = tims.size();
+  std::vector<TrialWork> all;
+  std::vector<std::pair<size_t, size_t>> where;  // (dm, acc index) of each trial
+  for (size_t d = 0; d < ndm; ++d)
+    for (size_t j = 0; j < accs[d].size(); ++j) {
+      all.push_back(TrialWork{tims[d], accs[d][j], means[d], stds[d], dms[d], dm_idx[d]});
+      where.emplace_back(d, j);
+    }
+  std::vector<std::vector<std::vector<Candidate>>> per_trial(ndm);
+  for (size_t d = 0; d < ndm; ++d) per_trial[d].resize(accs[d].size());
+  const uint32_t B = eng.batch();
+  size_t next = 0, done = 0;
+  std::vector<size_t> starts;  // first trial of each submitted batch
+  std::vector<std::vector<Candidate>> got;
+  while (done < all.size()) {
+    while (next < all.size() && eng.in_flight() < 2) {
+      const uint32_t n = (uint32_t)std::min<size_t>(B, all.size() - next);
+      eng.submit(&all[next], n);
+      starts.push_back(next);
+      next += n;
+    }
+    if (!eng.collect(got)) break;
+    const size_t first = starts.front();
+    starts.erase(starts.begin());
+    for (size_t t = 0; t < got.size(); ++t) {
+      const auto& w = where[first + t];
+      per_trial[w.first][w.second] = std::move(got[t]);
+    }
+    done += got.size();
+  }
+  const float tobs = (unsigned int)a.size * a.tsamp;
+  per_dm.assign(ndm, {});
+  for (size_t d = 0; d < ndm; ++d) {
+    std::vector<Candidate> accel_trial_cands;
+    for (auto& v : per_trial[d]) accel_trial_cands.insert(accel_trial_cands.end(), v.begin(), v.end());
+    AccelerationDistiller acc_still(tobs, a.freq_tol, true);
+    per_dm[d] = acc_still.distill(accel_trial_cands);
+  }
+}
+
+}  // namespace search
+}  // namespace psx
+
+namespace {
+bool engine_options(const psx_batch_opts* opts, EngineOptions& eo, psx::fft::Decomp& forced) {
+  if (opts) {
+    eo.batch = opts->batch ? opts->batch : eo.batch;
+    eo.cand_capacity = opts->cand_capacity ? opts->cand_capacity : eo.cand_capacity;
+    eo.use_graph = opts->use_graph != 0;
+    if (opts->fft_npass > 0) {
+      forced.npass = opts->fft_npass;
+      for (int q = 0; q < 3; ++q) {
+        forced.L[q] = opts->fft_len[q];
+        forced.C[q] = opts->fft_cols[q];
+      }
+      eo.decomp = &forced;
+    }
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
+                                 uint64_t nsamps, const float* accs, uint32_t naccs, int reps, double* stage_ms,
+                                 int* nstage, char* desc, size_t cap) {
+  int rc = psx::search::validate_args(args);
+  if (rc) return rc;
+  PSX_REQUIRE(d_series && accs && naccs > 0 && reps > 0 && stage_ms && nstage, "psx_batch_profile: bad arguments");
+  hipStream_t s = nullptr;
+  float* d_tim = nullptr;
+  try {
+    EngineOptions eo;
+    psx::fft::Decomp forced;
+    engine_options(opts, eo, forced);
+    PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
+    PSX_CHECK_HIP(hipMalloc(&d_tim, sizeof(float) * args->size));
+    float mean, sd;
+    eng->prologue(d_series, nsamps, d_tim, &mean, &sd);
+    const uint32_t n = std::min(naccs, eng->batch());
+    std::vector<TrialWork> tw(n);
+    for (uint32_t t = 0; t < n; ++t) tw[t] = TrialWork{d_tim, accs[t], mean, sd, 0.0f, 0};
+    psx::batch::StageTimes st;
+    eng->profile(tw.data(), n, reps, st);
+    const int ns = (int)st.ms.size();
+    for (int k = 0; k < ns && k < *nstage; ++k) stage_ms[k] = st.ms[k];
+    *nstage = ns;
+    if (desc && cap) {
+      strncpy(desc, st.desc.c_str(), cap - 1);
+      desc[cap - 1] = 0;
+    }
+    eng.reset();
+    (void)hipFree(d_tim);
+    (void)hipStreamDestroy(s);
+    return psx::ok();
+  } catch (const std::exception& e) {
+    if (d_tim) (void)hipFree(d_tim);
+    if (s) (void)hipStreamDestroy(s);
+    return psx::fail(PSX_ERR_HIP, "psx_batch_profile: %s", e.what());
+  }
+}
+
+extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opts* opts, const float* series,
+                                int series_on_device, uint64_t nsamps, const float* dm_list, uint32_t ndm,
+                                const float* accs, uint32_t naccs, psx_candidate* out, uint32_t cap, uint32_t* count,
+                                double* seconds) {
+  int rc = psx::search::validate_args(args);
+  if (rc) return rc;
+  PSX_REQUIRE(series && dm_list && ndm > 0 && nsamps > 0, "psx_batch_search: bad series");
+  PSX_REQUIRE(!accs || naccs > 0, "psx_batch_search: empty acceleration list");
+  hipStream_t s = nullptr;
+  float* d_tims = nullptr;
+  float* d_stage = nullptr;
+  try {
+    EngineOptions eo;
+    psx::fft::Decomp forced;
+    engine_options(opts, eo, forced);
+    PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const uint64_t N = args->size;
+    std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
+    PSX_CHECK_HIP(hipMalloc(&d_tims, sizeof(float) * N * ndm));
+    const uint64_t nreal = nsamps;
+    auto t0 = clk::now();
+    std::vector<const float*> tims(ndm);
+    std::vector<float> means(ndm), stds(ndm), dms(dm_list, dm_list + ndm);
+    std::vector<int> idx(ndm);
+    std::vector<std::vector<float>> acc_lists(ndm);
+    AccelerationPlan plan(args->acc_start, args->acc_end, args->acc_tol, args->acc_pulse_width * 1e-6, N,
+                          args->tsamp, args->cfreq, args->ch_bw);
+    if (!series_on_device) PSX_CHECK_HIP(hipMalloc(&d_stage, sizeof(float) * nreal));
+    for (uint32_t d = 0; d < ndm; ++d) {
+      float* tim = d_tims + (uint64_t)d * N;
+      const float* src = series + (uint64_t)d * nreal;
+      if (!series_on_device) {
+        PSX_CHECK_HIP(hipMemcpyAsync(d_stage, src, sizeof(float) * nreal, hipMemcpyHostToDevice, s));
+        src = d_stage;
+      }
+      eng->prologue(src, nreal, tim, &means[d], &stds[d]);
+      tims[d] = tim;
+      idx[d] = (int)d;
+      if (accs)
+        acc_lists[d].assign(accs, accs + naccs);
+      else
+        plan.generate_accel_list(dms[d], args->cdm, acc_lists[d]);
+    }
+    PSX_CHECK_HIP(hipStreamSynchronize(s));
+    auto t1 = clk::now();
+    std::vector<std::vector<Candidate>> per_dm;
+    psx::search::run_trials(*eng, *args, tims, means, stds, dms, idx, acc_lists, per_dm);
+    auto t2 = clk::now();
+    std::vector<Candidate> flat;
+    for (auto& v : per_dm) flat.insert(flat.end(), v.begin(), v.end());
+    if (seconds) {
+      seconds[0] = secs(t0, t1);
+      seconds[1] = secs(t1, t2);
+      seconds[2] = secs(t0, t2);
+    }
+    eng.reset();
+    (void)hipFree(d_tims);
+    if (d_stage) (void)hipFree(d_stage);
+    (void)hipStreamDestroy(s);
+    return psx::search::flatten(flat, out, cap, count) == PSX_OK ? psx::ok() : PSX_ERR_OVERFLOW;
+  } catch (const std::exception& e) {
+    if (d_tims) (void)hipFree(d_tims);
+    if (d_stage) (void)hipFree(d_stage);
+    if (s) (void)hipStreamDestroy(s);
+    return psx::fail(PSX_ERR_HIP, "psx_batch_search: %s", e.what());
+  }
+}

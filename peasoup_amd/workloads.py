@@ -143,7 +143,25 @@ def generate_host(w: Workload, trials=None):
     return out
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("batch", C.c_uint32), ("cand_capacity", C.c_uint32), ("use_graph", C.c_int32),
+                ("fft_npass", C.c_int32), ("fft_len", C.c_int32 * 3), ("fft_cols", C.c_int32 * 3)]
+
+
+def make_batch_opts(batch=16, cand_capacity=0, use_graph=True, decomp=None):
+    o = BatchOpts(batch, cand_capacity, 1 if use_graph else 0, 0)
+    if decomp:
+        o.fft_npass = len(decomp)
+        for q, (L, Cc) in enumerate(decomp):
+            o.fft_len[q] = L
+            o.fft_cols[q] = Cc
+    return o
+
+
 P.register_signatures({
+    "psx_batch_search": (C.c_int, [C.POINTER(SearchArgs), C.POINTER(BatchOpts), C.c_void_p, C.c_int, C.c_uint64,
+                                   C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                   C.POINTER(C.c_uint32), C.c_void_p]),
     "psx_worker_search": (C.c_int, [C.POINTER(SearchArgs), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
                                     C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
                                     C.c_void_p]),
@@ -163,3 +181,34 @@ def worker_search(args, series, dm_list, accs=None, cap=1 << 20):
                                       0 if a is None else a.size, out.ctypes.data, cap, C.byref(cnt),
                                       secs.ctypes.data))
     return out[:cnt.value], secs
+
+
+def batch_search(args, series, dm_list, accs=None, opts=None, nsamps=None, cap=1 << 20):
+    """Batched fused search.  series: host array [ndm, nsamps] or a DeviceArray
+    holding ndm x nsamps samples (then nsamps is required)."""
+    dm_list = np.ascontiguousarray(dm_list, np.float32)
+    on_dev = isinstance(series, P.DeviceArray)
+    if on_dev:
+        ptr, ns = series.ptr, nsamps
+    else:
+        series = np.ascontiguousarray(series, np.float32)
+        ptr, ns = series.ctypes.data, series.shape[-1]
+    out = np.zeros(cap, CAND_DTYPE)
+    cnt = C.c_uint32()
+    secs = np.zeros(3, np.float64)
+    a = None if accs is None else np.ascontiguousarray(accs, np.float32)
+    o = opts if opts is not None else make_batch_opts()
+    P.check(P.lib().psx_batch_search(C.byref(args), C.byref(o), ptr, 1 if on_dev else 0, ns, dm_list.ctypes.data,
+                                     dm_list.size, None if a is None else a.ctypes.data, 0 if a is None else a.size,
+                                     out.ctypes.data, cap, C.byref(cnt), secs.ctypes.data))
+    return out[:cnt.value], secs
+
+
+def generate_device(w: Workload, trials=None):
+    """Series of a workload generated straight into one device buffer."""
+    trials = list(range(w.ndm)) if trials is None else list(trials)
+    gen = SeriesGenerator(w.n_real)
+    d = P.DeviceArray((len(trials), w.n_real))
+    for k, t in enumerate(trials):
+        gen.generate(d.offset(k * w.n_real), w.n_real, w.tsamp, w.seed, t, w.red_amp, w.injections)
+    return d

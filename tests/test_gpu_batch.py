@@ -1,0 +1,87 @@
+"""The batched, fused, graph-launched search returns the plain Worker's
+candidates exactly: every thresholded value is computed with the same
+arithmetic, and host clustering/distilling is unchanged.  Results must not
+depend on how trials are grouped into batches (alone vs in the mix)."""
+import numpy as np
+import pytest
+
+from candmatch import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    key = lambda c: (int(c["dm_idx"]), float(c["acc"]), int(c["nh"]), float(c["freq"]), float(c["snr"]),
+                     int(c["nassoc"]))
+    return sorted(map(key, a)) == sorted(map(key, b))
+
+
+@pytest.fixture(scope="module")
+def cfg1(psx):
+    import peasoup_amd.workloads as W
+    w = W.config("cpu8x2^17")
+    return w, W.generate_host(w), W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+
+
+@pytest.mark.parametrize("batch,graph", [(16, True), (3, True), (1, False), (5, False)])
+def test_batched_equals_worker(psx, cfg1, batch, graph):
+    import peasoup_amd.workloads as W
+    w, series, accs = cfg1
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(batch, use_graph=graph))
+    assert same(got, ref)
+
+
+def test_batched_vs_oracle(psx, orc, cfg1):
+    import peasoup_amd.workloads as W
+    w, series, accs = cfg1
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs)
+    ref = []
+    for d in range(w.ndm):
+        p = orc.make_params(w.size, np.float32(w.tsamp), dm=float(d), dm_idx=d, nharmonics=w.nharmonics)
+        ref.append(orc.search_dm(series[d], p, accs, nthreads=4)[0])
+    assert not compare(got, np.concatenate(ref), w.min_snr)
+
+
+def test_batched_device_input_and_plan(psx, cfg1):
+    import peasoup_amd.workloads as W
+    w, series, accs = cfg1
+    d = psx.DeviceArray.from_numpy(series)
+    a, _ = W.batch_search(w.search_args(), d, w.dm_list(), accs, nsamps=w.size)
+    b, _ = W.batch_search(w.search_args(), series, w.dm_list(), None)
+    assert same(a, b)
+
+
+def test_batched_2pow23_matches_worker(psx):
+    """Headline length 2^23 at 76 us with red noise and an injected pulsar."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("t", 2, 1 << 23, 76e-6, 99, 5.0, -100, 100,
+                   injections=[W.Injection(1, 0.01234, 35.0, 25.0, 0.3)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)[::16]
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs)
+    got, secs = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(4))
+    assert same(got, ref)
+    assert np.any((got["dm_idx"] == 1) & (got["snr"] > 10))
+
+
+@pytest.mark.parametrize("n", [3 * (1 << 21), 1 << 22])
+def test_batched_uneven_sizes_match_worker(psx, n):
+    import peasoup_amd.workloads as W
+    w = W.Workload("t", 1, n, 76e-6, 7, 5.0, -50, 50, injections=[W.Injection(0, 0.0071, -20.0, 20.0)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)[::6]
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(3))
+    assert same(got, ref)
+
+
+def test_batched_overflow_is_an_error(psx):
+    """A threshold below the noise floor floods the candidate buffer: the
+    search must fail loudly rather than drop candidates."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("t", 1, 1 << 16, 64e-6, 5, 0.0, 0, 0)
+    series = W.generate_host(w)
+    args = W.make_search_args(w.size, w.tsamp, acc_start=0, acc_end=0, min_snr=-10.0)
+    with pytest.raises(psx.PsxError):
+        W.batch_search(args, series, w.dm_list(), None, W.make_batch_opts(1, cand_capacity=10000))

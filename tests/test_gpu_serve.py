@@ -1,0 +1,92 @@
+"""Serving: continuous batching across requests of mixed sizes.
+
+Each request's candidates must be identical alone and in the mix, and equal
+to the batched search of the same input; admission respects the HBM budget;
+a failing request does not disturb the others."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    key = lambda c: (int(c["dm_idx"]), float(c["acc"]), int(c["nh"]), float(c["freq"]), float(c["snr"]),
+                     int(c["nassoc"]))
+    return sorted(map(key, a)) == sorted(map(key, b))
+
+
+def make_requests():
+    import peasoup_amd.workloads as W
+    specs = [
+        W.Workload("a", 2, 1 << 17, 64e-6, 11, 0.0, -50, 50, injections=[W.Injection(1, 5.3e-3, 10.0, 18.0)]),
+        W.Workload("b", 3, 1 << 16, 64e-6, 12, 3.0, -100, 100, injections=[W.Injection(0, 3.1e-3, -30.0, 20.0)]),
+        W.Workload("c", 1, 1 << 19, 64e-6, 13, 5.0, -50, 50, injections=[W.Injection(0, 11e-3, 5.0, 25.0)]),
+        W.Workload("d", 2, 1 << 17, 64e-6, 14, 0.0, -200, 200),
+    ]
+    return [(w, W.generate_host(w)) for w in specs]
+
+
+@pytest.fixture(scope="module")
+def reqs(psx):
+    return make_requests()
+
+
+def test_alone_equals_in_the_mix(psx, reqs):
+    import peasoup_amd.workloads as W
+    from peasoup_amd.serve import Server
+    alone = []
+    for w, series in reqs:
+        with Server(64 << 30, batch=8) as srv:
+            rid = srv.submit(w.search_args(), series, w.dm_list())
+            st = srv.wait(rid)
+            assert st["state"] == "done", st
+            alone.append(srv.result(rid))
+        direct, _ = W.batch_search(w.search_args(), series, w.dm_list(), None, W.make_batch_opts(8))
+        assert same(alone[-1], direct), w.name
+    with Server(64 << 30, batch=8) as srv:
+        ids = [srv.submit(w.search_args(), s, w.dm_list()) for w, s in reqs]
+        # a second copy of request a, submitted while the others run
+        ids.append(srv.submit(reqs[0][0].search_args(), reqs[0][1], reqs[0][0].dm_list()))
+        sts = [srv.wait(i) for i in ids]
+        assert all(s["state"] == "done" for s in sts), sts
+        mixed = [srv.result(i) for i in ids]
+        st = srv.stats()
+        assert st["completed"] == len(ids) and st["failed"] == 0
+        assert st["engines"] >= 3
+    for k in range(len(reqs)):
+        assert same(mixed[k], alone[k]), reqs[k][0].name
+    assert same(mixed[-1], alone[0])
+
+
+def test_budget_admission_and_rejection(psx, reqs):
+    from peasoup_amd.serve import Server
+    w, series = reqs[0]
+    one = 2 * w.size * 4  # resident series of one 2-DM request
+    # engine buffers are counted too: allow one engine plus one request only
+    import ctypes as C
+    import peasoup_amd as P
+    budget = 3 * one + 600 * (1 << 20)
+    with Server(budget, batch=8, max_active=0) as srv:
+        ids = [srv.submit(w.search_args(), series, w.dm_list()) for _ in range(3)]
+        for i in ids:
+            assert srv.wait(i)["state"] == "done"
+        assert srv.stats()["used_bytes"] <= budget
+    with Server(1 << 20, batch=8) as srv:
+        with pytest.raises(P.PsxError) as e:
+            srv.submit(w.search_args(), series, w.dm_list())
+        assert e.value.code == 6  # PSX_ERR_BUDGET
+
+
+def test_failing_request_is_isolated(psx, reqs):
+    import peasoup_amd.workloads as W
+    from peasoup_amd.serve import Server
+    w, series = reqs[0]
+    bad = W.make_search_args(w.size, w.tsamp, acc_start=-5e9, acc_end=5e9)  # |a tsamp / 2c| n >= 1
+    with Server(64 << 30, batch=8) as srv:
+        good1 = srv.submit(w.search_args(), series, w.dm_list())
+        badid = srv.submit(bad, series, w.dm_list(), accs=np.array([5e9], np.float32))
+        good2 = srv.submit(w.search_args(), series, w.dm_list())
+        assert srv.wait(good1)["state"] == "done"
+        assert srv.wait(good2)["state"] == "done"
+        assert srv.wait(badid)["state"] == "failed"
+        assert same(srv.result(good1), srv.result(good2))
