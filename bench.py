@@ -84,6 +84,97 @@ def cpu_baseline(O, W, w, series0, accs, threads, target_s=20.0):
                       (t_pro, n, len(accs), t_acc, len(accs))}
 
 
+def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16, log=log):
+    """Bounded Poisson stream of segment-search requests through the server.
+
+    Each request is one beam: DM count uniform in {2..8} (the serving
+    configuration's {16..256} scaled by 1/32 so the stream runs in seconds),
+    fft size uniform in {2^20..2^24}, tsamp 64 us, N(0,1) + red noise with 0-2
+    injected pulsars, accel range +-50 or +-200 m/s^2; admitted against an
+    HBM budget; latency = completion - arrival on the library clock."""
+    from peasoup_amd.serve import Server
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for k in range(nreq):
+        t += rng.exponential(1.0 / rate)
+        n = int(1 << rng.integers(20, 25))
+        ndm = int(rng.integers(2, 9))
+        acc = 50.0 if rng.random() < 0.5 else 200.0
+        w = W.Workload("req%d" % k, ndm, n, 64e-6, seed + k + 1, 5.0, -acc, acc)
+        for _ in range(int(rng.integers(0, 3))):
+            w.injections.append(W.Injection(int(rng.integers(0, ndm)), float(rng.uniform(1.5e-3, 0.2)),
+                                            float(rng.uniform(-acc, acc) * 0.8), float(rng.uniform(8, 30)),
+                                            float(rng.random())))
+        reqs.append((t, w, W.generate_device(w)))
+    P.synchronize()
+    ntrials = sum(w.ndm * W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi).size for _, w, _ in reqs)
+    with Server(budget, batch=batch) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for ta, w, d in reqs:
+            dt = ta - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(w.search_args(), d, w.dm_list(), nsamps=w.size))
+        sts = [srv.wait(i, timeout=600) for i in ids]
+        wall = time.perf_counter() - t0
+        for i in ids:
+            srv.result(i)
+        stats = srv.stats()
+    bad = [s for s in sts if s["state"] != "done"]
+    if bad:
+        raise RuntimeError("serving: %d requests failed: %s" % (len(bad), bad[:2]))
+    lat = np.array([s["latency"] for s in sts])
+    first = min(s["t_submit"] for s in sts)
+    last = max(s["t_done"] for s in sts)
+    res = {"requests": nreq, "arrival_rate_per_s": rate, "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 4), "max_latency_s": round(float(lat.max()), 4),
+           "trials": int(ntrials), "trials_per_s": round(ntrials / (last - first), 1),
+           "wall_s": round(wall, 2), "engines": stats["engines"], "hbm_budget_bytes": int(budget),
+           "stream": "Poisson %.1f req/s; DMs U{2..8}; fft size U{2^20..2^24}; tsamp 64 us; accel +-50 or "
+                     "+-200 m/s^2; red noise + 0-2 pulsars" % rate}
+    log("serving", res)
+    return res
+
+
+def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, log=log):
+    """Segmented searches of uneven length: ndm DM trials, an equal mix of fft
+    sizes 2^22, 3*2^21 and 2^23 at 76 us (each segment 90% of its fft size,
+    so the series are zero padded after baseline removal), red noise and
+    injections, accel -50..+50 m/s^2.  The three size classes are submitted
+    together to the server, whose per-size engines run concurrently."""
+    from peasoup_amd.serve import Server
+    sizes = [1 << 22, 3 << 21, 1 << 23]
+    per = [ndm // 3 + (1 if k < ndm % 3 else 0) for k in range(3)]
+    reqs = []
+    for k, (n, nd) in enumerate(zip(sizes, per)):
+        w = W.Workload("seg%d" % n, nd, n, 76e-6, seed + k, 5.0, -50.0, 50.0, nsamps=int(n * 0.9))
+        rng = np.random.default_rng(seed + k)
+        for t in rng.choice(nd, 2, replace=False):
+            w.injections.append(W.Injection(int(t), float(rng.uniform(2e-3, 0.2)), float(rng.uniform(-40, 40)),
+                                            float(rng.uniform(8, 30)), float(rng.random())))
+        reqs.append((w, W.generate_device(w)))
+    P.synchronize()
+    ntrials = sum(w.ndm * W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi).size for w, _ in reqs)
+    with Server(200e9, batch=batch) as srv:
+        # warm-up: engines for every size class
+        for w, d in reqs:
+            srv.wait(srv.submit(w.search_args(), d, w.dm_list()[:1], nsamps=w.n_real))
+        t0 = time.perf_counter()
+        ids = [srv.submit(w.search_args(), d, w.dm_list(), nsamps=w.n_real) for w, d in reqs]
+        sts = [srv.wait(i, timeout=600) for i in ids]
+        dt = time.perf_counter() - t0
+        ncand = sum(len(srv.result(i)) for i in ids)
+    if any(s["state"] != "done" for s in sts):
+        raise RuntimeError("uneven batch failed: %s" % sts)
+    res = {"trials": int(ntrials), "dm_trials": ndm, "seconds": round(dt, 3), "trials_per_s": round(ntrials / dt, 1),
+           "candidates": ncand, "workload": "%d DM trials, equal mix of fft sizes 2^22, 3*2^21, 2^23 (segments "
+                                            "90%% of the fft size, zero padded), 76 us, accel -50..+50" % ndm}
+    log("uneven", res)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +186,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for rocprofv3: a few fused batches of the workload plus a 1 GiB "
+                         "copy kernel (calibrates FETCH_SIZE/WRITE_SIZE), no timing line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (DM trials would shard across "
@@ -121,6 +218,19 @@ def main():
     opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"))
     log("workload %s: %d DMs x %d accels = %d trials, batch %d, decomp %s" %
         (w.name, w.ndm, accs.size, ntrials, batch, tuned.get("decomp")))
+
+    if a.profile_run:
+        w.ndm = 1
+        d_series = W.generate_device(w)
+        # copy kernel of known size: 2^28 floats read and written (1 GiB each way)
+        src = P.DeviceArray(1 << 28)
+        dst = P.DeviceArray(1 << 28)
+        for _ in range(3):
+            P.check(P.lib().psx_convert_to_f32(src.ptr, P.PSX_F32, dst.ptr, 1 << 28, None))
+        P.synchronize()
+        stage_ms, desc = profile_stages(P, W, args_c, opts, d_series.ptr, w.size, accs[:batch], reps=5)
+        log("profile run: plan %s, stage ms %s" % (desc, stage_ms))
+        return
 
     t0 = time.perf_counter()
     d_series = W.generate_device(w)
@@ -197,6 +307,12 @@ def main():
         result["cpu_baseline"] = cb
         result["speedup_vs_cpu"] = round(value / cb["value"], 1)
         log("cpu baseline", cb)
+    if not a.no_uneven or not a.no_serve:
+        del d_series
+    if not a.no_uneven:
+        result["uneven_batched"] = uneven_batched(P, W, batch=batch)
+    if not a.no_serve:
+        result["serving"] = serve_stream(P, W, nreq=a.serve_requests, batch=batch)
     print(json.dumps(result), flush=True)
 
 

@@ -138,8 +138,9 @@ def device_info(dev=0):
     name = C.create_string_buffer(256)
     hbm, cus, arch = C.c_uint64(), C.c_int(), C.c_int()
     check(lib().psx_device_info(dev, name, 256, C.byref(hbm), C.byref(cus), C.byref(arch)))
-    return {"name": name.value.decode(), "hbm_bytes": hbm.value, "cu_count": cus.value,
-            "gfx": "gfx%x" % arch.value}
+    gfx = "gfx%x" % arch.value
+    nm = name.value.decode() or "%s-%dCU" % (gfx, cus.value)
+    return {"name": nm, "hbm_bytes": hbm.value, "cu_count": cus.value, "gfx": gfx}
 
 
 class DeviceArray:

@@ -22,8 +22,35 @@ namespace {
 
 constexpr unsigned kB = 256;
 
-// untangle + form_interpolated + normalise (src/kernels.cu:238-259,496-507)
-// for bins 2q and 2q+1 of trial blockIdx.y.
+// untangle + form_interpolated + normalise (src/kernels.cu:238-259,496-507).
+// Lane l of a wave owns lower bin k = base + l (k <= M/2) and its mirror
+// M-k.  The pair (Z[k], Z[M-k]) untangles into both X[k] and X[M-k] (the two
+// halves of the plain untangle pass, same arithmetic); loads of Z[k] and
+// Z[M-k] are coalesced (the second in descending order).  Interbinning needs
+// X[k-1] and X[M-k-1] = upper X of pair k+1: both come from the neighbouring
+// lane by a wave shuffle, and the wave's edge lanes untangle that pair
+// themselves.
+__device__ inline float normalised(float2 x, float2 prev, float mean_n, float std_n) {
+  float v = psx::interbin(x, prev);
+  v -= mean_n;
+  v /= std_n;
+  return v;
+}
+
+// Lower (X[k]) or upper (X[M-k]) half of untangle pair k, 0 <= k <= M/2.
+__device__ inline float2 pair_half(const float2* __restrict__ z, uint32_t m, uint32_t k, const RootTable& rt,
+                                   bool upper) {
+  const float2 zk = z[k];
+  const float2 zm = z[k == 0 ? 0 : m - k];
+  const float2 w = root(rt, (uint64_t)k);
+  if (!upper || 2 * k == m) return untangle_one(zk, zm, w);
+  return untangle_one(zm, zk, make_float2(-w.x, w.y));
+}
+
+__device__ inline float2 shfl_f2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src), __shfl(v.y, src));
+}
+
 __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, float* __restrict__ P, uint64_t M,
                                                   int64_t pstride, RootTable rt, const TrialDesc* __restrict__ descs) {
   const uint32_t tr = blockIdx.y;
@@ -31,24 +58,31 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
   float* p = P + (int64_t)tr * pstride;
   const float mean_n = descs[tr].mean_n;
   const float std_n = descs[tr].std_n;
-  const uint64_t nb = M + 1;
-  const uint64_t npair = (nb + 1) / 2;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npair; q += stride) {
-    const uint64_t k = 2 * q;
-    const float2 xm1 = k > 0 ? untangled_bin(z, M, k - 1, rt) : make_float2(0.f, 0.f);
-    const float2 x0 = untangled_bin(z, M, k, rt);
-    float v0 = psx::interbin(x0, xm1);
-    v0 -= mean_n;
-    v0 /= std_n;
-    if (k + 1 < nb) {
-      const float2 x1 = untangled_bin(z, M, k + 1, rt);
-      float v1 = psx::interbin(x1, x0);
-      v1 -= mean_n;
-      v1 /= std_n;
-      reinterpret_cast<float2*>(p)[q] = make_float2(v0, v1);
-    } else {
-      p[k] = v0;
+  const uint32_t m = (uint32_t)M, half = m / 2;
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base <= half; base += stride) {
+    const uint32_t k = base + threadIdx.x;
+    const bool valid = k <= half;
+    float2 xl = make_float2(0.f, 0.f), xu = make_float2(0.f, 0.f);
+    if (valid) {
+      const float2 zk = z[k];
+      const float2 zm = z[k == 0 ? 0 : m - k];
+      const float2 w = root(rt, (uint64_t)k);
+      xl = untangle_one(zk, zm, w);
+      // W_n^(M-k) = -conj(W_n^k); the midpoint is its own mirror and keeps the
+      // lower formula, as the plain pass writes it
+      xu = (k == half) ? xl : untangle_one(zm, zk, make_float2(-w.x, w.y));
+    }
+    float2 xl_prev = shfl_f2(xl, lane > 0 ? lane - 1 : 0);
+    float2 xu_next = shfl_f2(xu, lane < 63 ? lane + 1 : 63);
+    if (valid) {
+      if (lane == 0) xl_prev = k == 0 ? make_float2(0.f, 0.f) : pair_half(z, m, k - 1, rt, false);
+      if (k < half) {
+        if (lane == 63) xu_next = pair_half(z, m, k + 1, rt, true);
+        p[m - k] = normalised(xu, xu_next, mean_n, std_n);
+      }
+      p[k] = normalised(xl, xl_prev, mean_n, std_n);
     }
   }
 }
@@ -61,35 +95,47 @@ template <int NF>
 __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, int64_t pstride, Windows w,
                                                    const TrialDesc* __restrict__ descs, Cand16* __restrict__ out,
                                                    uint32_t cap, uint32_t* __restrict__ count) {
+  constexpr int Q = 4;  // bins per thread per iteration (independent gather chains)
   const uint32_t tr = blockIdx.y;
   const float* p = P + (int64_t)tr * pstride;
-  const uint32_t slot = descs[tr].slot;
   const unsigned lane = threadIdx.x & 63u;
-  const int64_t step = (int64_t)gridDim.x * kB;
-  for (int64_t base = w.lo + (int64_t)blockIdx.x * kB; base < w.hi; base += step) {
-    const int64_t i = base + threadIdx.x;
-    const bool valid = i < w.hi;
-    float s[6];
-    s[0] = 0.f;
-    if (valid) {
-      float f[5];
-      psx::harmonic_sums<NF>(p, (uint64_t)i, f);
-      s[0] = p[i];
+  const int32_t step = (int32_t)(gridDim.x * kB * Q);
+  const int32_t hi = (int32_t)w.hi;
+  for (int32_t base = (int32_t)w.lo + (int32_t)(blockIdx.x * kB * Q); base < hi; base += step) {
+    float s[Q][6];
+    bool any = false;
 #pragma unroll
-      for (int k = 0; k < NF; ++k) s[k + 1] = f[k];
+    for (int q = 0; q < Q; ++q) {
+      const int32_t i = base + (int32_t)threadIdx.x + q * (int32_t)kB;
+      if (i < hi) {
+        float f[5];
+        psx::harmonic_sums32<NF>(p, (uint32_t)i, f);
+        s[q][0] = p[i];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) s[q][k + 1] = f[k];
+#pragma unroll
+        for (int nh = 0; nh <= NF; ++nh) any |= (i >= w.start[nh]) & (i < w.end[nh]) & (s[q][nh] > w.thr);
+      }
     }
+    // survivors are rare: one ballot decides whether this wave appends at all
+    if (__ballot(any) == 0) continue;
+    const uint32_t slot = descs[tr].slot;
 #pragma unroll
-    for (int nh = 0; nh <= NF; ++nh) {
-      const bool hit = valid && i >= w.start[nh] && i < w.end[nh] && s[nh] > w.thr;
-      const uint64_t mask = __ballot(hit);
-      if (mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        uint32_t b = 0;
-        if ((int)lane == leader) b = atomicAdd(count, (uint32_t)__popcll(mask));
-        b = __shfl(b, leader);
-        if (hit) {
-          const uint32_t pos = b + psx::mbcnt64(mask);
-          if (pos < cap) out[pos] = Cand16{slot, (uint32_t)nh, (int32_t)i, s[nh]};
+    for (int q = 0; q < Q; ++q) {
+      const int32_t i = base + (int32_t)threadIdx.x + q * (int32_t)kB;
+#pragma unroll
+      for (int nh = 0; nh <= NF; ++nh) {
+        const bool hit = (i < hi) && i >= w.start[nh] && i < w.end[nh] && s[q][nh] > w.thr;
+        const uint64_t mask = __ballot(hit);
+        if (mask) {
+          const int leader = __ffsll((long long)mask) - 1;
+          uint32_t b = 0;
+          if ((int)lane == leader) b = atomicAdd(count, (uint32_t)__popcll(mask));
+          b = __shfl(b, leader);
+          if (hit) {
+            const uint32_t pos = b + psx::mbcnt64(mask);
+            if (pos < cap) out[pos] = Cand16{slot, (uint32_t)nh, i, s[q][nh]};
+          }
         }
       }
     }
@@ -104,8 +150,8 @@ __global__ void k_zero_u32(uint32_t* p) {
 
 int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, const RootTable& rt,
                      const TrialDesc* d_descs, uint32_t ntr, hipStream_t s) {
-  const uint64_t npair = (M + 2) / 2;
-  const unsigned gx = psx::grid_for(npair, kB, 2048);
+  if (M < 8 || M > 0x7FFFFFFFull) return psx::fail(PSX_ERR_ARG, "spec_post: unsupported length");
+  const unsigned gx = psx::grid_for(M / 2 + 1, kB, 2048);
   k_spec_post<<<dim3(gx, ntr), kB, 0, s>>>(Z, P, M, pstride, rt, d_descs);
   PSX_CHECK_LAUNCH("k_spec_post");
   return PSX_OK;
@@ -114,7 +160,9 @@ int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, con
 int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_t nfolds, const TrialDesc* d_descs,
                       uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count, hipStream_t s) {
   if (w.hi <= w.lo) return PSX_OK;
-  const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), kB, 2048);
+  // 32-bit bin arithmetic: j*i < 32 * 2^27
+  if (w.hi > (1ll << 27)) return psx::fail(PSX_ERR_UNSUPPORTED, "harm_peaks: more than 2^27 bins");
+  const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), 4 * kB, 2048);
   const dim3 g(gx, ntr);
   switch (nfolds) {
     case 1: k_harm_peaks<1><<<g, kB, 0, s>>>(P, pstride, w, d_descs, out, cap, count); break;

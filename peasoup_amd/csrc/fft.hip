@@ -25,7 +25,23 @@ using namespace psx::fft;
 namespace {
 
 // ------------------------------------------------------------ pass kernel
-template <int S, int L, int C>
+// Pass modes fix, at compile time, which side of the pass has adjacent
+// columns (column-fast thread mapping, C*8 contiguous bytes per row) and
+// whether the four-step twiddle is applied on the way out:
+//   PM_CONTIG        contiguous rows in and out (single-pass transforms)
+//   PM_COLS_TW       strided columns in and out, twiddle W_M^(c*e) applied
+//   PM_ROWS_TO_COLS  contiguous rows in, strided columns out
+// RS: the first read goes through the resampler (batched search).
+enum PassMode { PM_CONTIG = 0, PM_COLS_TW = 1, PM_ROWS_TO_COLS = 2 };
+
+template <int MODE>
+struct ModeTraits {
+  static constexpr bool in_cf = MODE == PM_COLS_TW;
+  static constexpr bool out_cf = MODE != PM_CONTIG;
+  static constexpr bool tw4 = MODE == PM_COLS_TW;
+};
+
+template <int S, int L, int C, int MODE>
 __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds, const PassArgs& a, float2* out,
                                  unsigned col0, int t) {
   constexpr int P = points_per_thread(L);
@@ -35,7 +51,7 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds,
   constexpr int NB = P / R;
   constexpr int Ns = ns_before(L, S);
   constexpr bool LAST = (S == NSTG - 1);
-  const bool cf = (S == 0) ? (a.in_cf != 0) : (LAST ? (a.out_cf != 0) : false);
+  constexpr bool cf = (S == 0) ? ModeTraits<MODE>::in_cf : (LAST ? ModeTraits<MODE>::out_cf : false);
 
   // twiddle + butterfly
 #pragma unroll
@@ -54,6 +70,7 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds,
   if constexpr (!LAST) {
     constexpr int R2 = radix_at(L, S + 1);
     constexpr int NB2 = P / R2;
+    constexpr bool cf2 = (S + 1 == NSTG - 1) ? ModeTraits<MODE>::out_cf : false;
     if (S > 0) __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -65,7 +82,6 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds,
       for (int r = 0; r < R; ++r) lds[lds_index<L, C>(c, idxD + r * Ns)] = v[b * R + r];
     }
     __syncthreads();
-    const bool cf2 = (S + 1 == NSTG - 1) ? (a.out_cf != 0) : false;
 #pragma unroll
     for (int b = 0; b < NB2; ++b) {
       int c, j;
@@ -73,70 +89,82 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds,
 #pragma unroll
       for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds[lds_index<L, C>(c, j + r * (L / R2))];
     }
-    run_stage<S + 1, L, C>(v, lds, a, out, col0, t);
+    run_stage<S + 1, L, C, MODE>(v, lds, a, out, col0, t);
   } else {
+    // element offsets inside one transform are < n/2 <= 2^27: 32-bit math
+    const uint32_t ocol = (uint32_t)a.out_col, oel = (uint32_t)a.out_elem;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       int c, j;
       bf_coords<L, C, R, NT>(b, t, cf, c, j);
-      const uint64_t cg = (uint64_t)col0 + (uint64_t)c;
+      const uint32_t cg = col0 + (uint32_t)c;
+      const uint32_t o0 = cg * ocol + (uint32_t)j * oel;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int e = j + r * (L / R);
+        const uint32_t e = (uint32_t)(j + r * (L / R));
         float2 val = v[b * R + r];
-        if (a.tw4) val = cmul(val, tw4_lookup(a, cg * (uint64_t)e));
+        if constexpr (ModeTraits<MODE>::tw4) {
+          const uint32_t x = cg * e;
+          val = cmul(val, cmul(a.tw4_hi[x >> a.tw4_shift], a.tw4_lo[x & a.tw4_mask]));
+        }
         if (a.out_conj) val.y = -val.y;
-        out[(int64_t)cg * a.out_col + (int64_t)e * a.out_elem] = val;
+        out[o0 + (uint32_t)(r * (L / R)) * oel] = val;
       }
     }
   }
 }
 
-template <int L, int C>
+template <int L, int C, int MODE, bool RS>
 __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const PassArgs a) {
   constexpr int P = points_per_thread(L);
   constexpr int NT = C * L / P;
   constexpr int R0 = radix_at(L, 0);
   constexpr int NB0 = P / R0;
+  constexpr bool cf = ModeTraits<MODE>::in_cf;
   extern __shared__ float2 lds[];
   const unsigned blk = psx::xcd_logical_block(blockIdx.x, gridDim.x);
   const unsigned col0 = blk * C;
-  const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)blockIdx.z * a.in_tr;
   float2* out = a.out + (int64_t)blockIdx.y * a.out_sub + (int64_t)blockIdx.z * a.out_tr;
   const int t = threadIdx.x;
-  const bool cf = a.in_cf != 0;
+  const uint32_t icol = (uint32_t)a.in_col, iel = (uint32_t)a.in_elem;
   float2 v[P];
-  if (a.resample) {
-    // resampleII fused into the first read (src/kernels.cu:342-353)
+  if constexpr (RS) {
+    // resampleII fused into the first read (src/kernels.cu:342-353); the
+    // complex input m is the sample pair (2m, 2m+1) of the resampled series
     const TrialDesc d = a.descs[blockIdx.z];
-    const int64_t sub = (int64_t)blockIdx.y * a.in_sub;
+    const float* __restrict__ tim = d.tim;
+    const double af = d.af;
+    const uint32_t n = (uint32_t)a.rs_n;
+    const double dn = (double)n;
+    const uint32_t sub = (uint32_t)((int64_t)blockIdx.y * a.in_sub);
 #pragma unroll
     for (int b = 0; b < NB0; ++b) {
       int c, j;
       bf_coords<L, C, R0, NT>(b, t, cf, c, j);
-      const int64_t cg = (int64_t)col0 + c;
+      const uint32_t m0 = sub + (col0 + (uint32_t)c) * icol + (uint32_t)j * iel;
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
-        const uint64_t m = (uint64_t)(sub + cg * a.in_col + (int64_t)(j + r * (L / R0)) * a.in_elem);
-        v[b * R0 + r] = make_float2(d.tim[psx::resample_index(2 * m, a.rs_n, d.af)],
-                                    d.tim[psx::resample_index(2 * m + 1, a.rs_n, d.af)]);
+        const uint32_t m = m0 + (uint32_t)(r * (L / R0)) * iel;
+        v[b * R0 + r] = make_float2(tim[psx::resample_index32(2 * m, n, dn, af)],
+                                    tim[psx::resample_index32(2 * m + 1, n, dn, af)]);
       }
     }
   } else {
+    const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)blockIdx.z * a.in_tr;
 #pragma unroll
     for (int b = 0; b < NB0; ++b) {
       int c, j;
       bf_coords<L, C, R0, NT>(b, t, cf, c, j);
-      const int64_t cg = (int64_t)col0 + c;
+      const uint32_t o0 = (col0 + (uint32_t)c) * icol + (uint32_t)j * iel;
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
-        float2 x = in[cg * a.in_col + (int64_t)(j + r * (L / R0)) * a.in_elem];
+        float2 x = in[o0 + (uint32_t)(r * (L / R0)) * iel];
         if (a.in_conj) x.y = -x.y;
         v[b * R0 + r] = x;
       }
     }
   }
-  run_stage<0, L, C>(v, lds, a, out, col0, t);
+  run_stage<0, L, C, MODE>(v, lds, a, out, col0, t);
 }
 
 // --------------------------------------------------------- R2C untangle
@@ -192,36 +220,40 @@ __global__ void __launch_bounds__(256) k_tangle(const float2* __restrict__ X, fl
 
 // ------------------------------------------------- kernel registry
 struct KernelInfo {
-  int L, C;
+  int L, C, mode;
+  bool rs;
   void (*fn)(const PassArgs);
   int threads;
   int lds;
 };
 
-template <int L, int C>
-constexpr KernelInfo kinfo() {
-  return KernelInfo{L, C, &k_fft_pass<L, C>, C * L / points_per_thread(L), lds_bytes(L, C)};
+template <int L, int C, int MODE, bool RS>
+constexpr KernelInfo kinfo1() {
+  return KernelInfo{L, C, MODE, RS, &k_fft_pass<L, C, MODE, RS>, C * L / points_per_thread(L), lds_bytes(L, C)};
 }
+// single-pass kernels (C = 1): contiguous, plain and resampling
+#define PSX_K1(L) kinfo1<L, 1, PM_CONTIG, false>(), kinfo1<L, 1, PM_CONTIG, true>()
+// column-tile kernels: strided in/out with twiddle (plain and resampling)
+// and rows-to-columns
+#define PSX_KC(L, C) \
+  kinfo1<L, C, PM_COLS_TW, false>(), kinfo1<L, C, PM_COLS_TW, true>(), kinfo1<L, C, PM_ROWS_TO_COLS, false>()
 
 const KernelInfo kKernels[] = {
-    // single-pass (C = 1)
-    kinfo<8, 1>(), kinfo<16, 1>(), kinfo<32, 1>(), kinfo<64, 1>(), kinfo<128, 1>(), kinfo<256, 1>(),
-    kinfo<512, 1>(), kinfo<1024, 1>(), kinfo<2048, 1>(), kinfo<4096, 1>(),
-    kinfo<6, 1>(), kinfo<12, 1>(), kinfo<24, 1>(), kinfo<48, 1>(), kinfo<96, 1>(), kinfo<192, 1>(),
-    kinfo<384, 1>(), kinfo<768, 1>(), kinfo<1536, 1>(), kinfo<3072, 1>(),
-    // multi-pass column tiles
-    kinfo<32, 16>(), kinfo<64, 16>(), kinfo<128, 16>(), kinfo<256, 16>(), kinfo<512, 16>(), kinfo<1024, 16>(),
-    kinfo<64, 8>(), kinfo<128, 8>(), kinfo<256, 8>(), kinfo<512, 8>(), kinfo<1024, 8>(), kinfo<2048, 8>(),
-    kinfo<256, 4>(), kinfo<512, 4>(), kinfo<1024, 4>(), kinfo<2048, 4>(), kinfo<4096, 4>(),
-    kinfo<1024, 2>(), kinfo<2048, 2>(), kinfo<4096, 2>(),
-    kinfo<48, 16>(), kinfo<96, 16>(), kinfo<192, 16>(), kinfo<384, 16>(), kinfo<768, 16>(),
-    kinfo<96, 8>(), kinfo<192, 8>(), kinfo<384, 8>(), kinfo<768, 8>(), kinfo<1536, 8>(),
-    kinfo<384, 4>(), kinfo<768, 4>(), kinfo<1536, 4>(), kinfo<3072, 4>(),
+    PSX_K1(8), PSX_K1(16), PSX_K1(32), PSX_K1(64), PSX_K1(128), PSX_K1(256), PSX_K1(512), PSX_K1(1024),
+    PSX_K1(2048), PSX_K1(4096), PSX_K1(6), PSX_K1(12), PSX_K1(24), PSX_K1(48), PSX_K1(96), PSX_K1(192),
+    PSX_K1(384), PSX_K1(768), PSX_K1(1536), PSX_K1(3072),
+    PSX_KC(32, 16), PSX_KC(64, 16), PSX_KC(128, 16), PSX_KC(256, 16), PSX_KC(512, 16), PSX_KC(1024, 16),
+    PSX_KC(64, 8), PSX_KC(128, 8), PSX_KC(256, 8), PSX_KC(512, 8), PSX_KC(1024, 8), PSX_KC(2048, 8),
+    PSX_KC(256, 4), PSX_KC(512, 4), PSX_KC(1024, 4), PSX_KC(2048, 4), PSX_KC(4096, 4),
+    PSX_KC(1024, 2), PSX_KC(2048, 2), PSX_KC(4096, 2),
+    PSX_KC(48, 16), PSX_KC(96, 16), PSX_KC(192, 16), PSX_KC(384, 16), PSX_KC(768, 16),
+    PSX_KC(96, 8), PSX_KC(192, 8), PSX_KC(384, 8), PSX_KC(768, 8), PSX_KC(1536, 8),
+    PSX_KC(384, 4), PSX_KC(768, 4), PSX_KC(1536, 4), PSX_KC(3072, 4),
 };
 
-const KernelInfo* find_kernel(int L, int C) {
+const KernelInfo* find_kernel(int L, int C, int mode = -1, bool rs = false) {
   for (const KernelInfo& k : kKernels)
-    if (k.L == L && k.C == C) return &k;
+    if (k.L == L && k.C == C && (mode < 0 || (k.mode == mode && k.rs == rs))) return &k;
   return nullptr;
 }
 
@@ -373,6 +405,7 @@ bool decomp_supported(const Decomp& d) {
 
 struct PassLaunch {
   const KernelInfo* k;
+  const KernelInfo* krs;  // resampling variant of a first pass (batched search)
   PassArgs a;           // in/out pointers filled at exec time from src/dst
   int src, dst;         // 0 = user input, 1 = work, 2 = user output
   unsigned gx, gy, gz;
@@ -441,9 +474,10 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
   const int64_t trM = (int64_t)M;                // complex stride of work / real-as-complex buffers
   const int64_t trNB = (int64_t)(M + 1);         // complex stride of spectra
   const int64_t in_tr = c2r ? trM : trM;         // passes read z (R2C: input as complex) or work
-  auto make = [&](int q, int src, int dst) {
+  auto make = [&](int q, int src, int dst, int mode) {
     PassLaunch L{};
-    L.k = find_kernel(d.L[q], d.C[q]);
+    L.k = find_kernel(d.L[q], d.C[q], mode, false);
+    L.krs = find_kernel(d.L[q], d.C[q], mode, true);
     L.src = src;
     L.dst = dst;
     PassArgs& a = L.a;
@@ -460,19 +494,19 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
     return L;
   };
   if (d.npass == 1) {
-    PassLaunch L = make(0, c2r ? 1 : 0, 2);
+    PassLaunch L = make(0, c2r ? 1 : 0, 2, PM_CONTIG);
     L.a.in_col = 0; L.a.in_elem = 1; L.a.in_sub = 0;
     L.a.out_col = 0; L.a.out_elem = 1; L.a.out_sub = 0;
     L.gx = 1; L.gy = 1; L.gz = p->batch;
     p->passes.push_back(L);
   } else if (d.npass == 2) {
     const int64_t M2 = d.L[0], M1 = d.L[1];
-    PassLaunch A = make(0, c2r ? 1 : 0, 1);
+    PassLaunch A = make(0, c2r ? 1 : 0, 1, PM_COLS_TW);
     A.a.in_col = 1; A.a.in_elem = M1; A.a.in_sub = 0;
     A.a.out_col = 1; A.a.out_elem = M1; A.a.out_sub = 0;
     A.a.in_cf = 1; A.a.out_cf = 1;
     A.gx = (unsigned)(M1 / d.C[0]); A.gy = 1; A.gz = p->batch;
-    PassLaunch B = make(1, 1, 2);
+    PassLaunch B = make(1, 1, 2, PM_ROWS_TO_COLS);
     B.a.in_col = M1; B.a.in_elem = 1; B.a.in_sub = 0;
     B.a.out_col = 1; B.a.out_elem = M2; B.a.out_sub = 0;
     B.a.in_cf = 0; B.a.out_cf = 1;
@@ -481,17 +515,17 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
     p->passes.push_back(B);
   } else {
     const int64_t Q = d.L[0], M2 = d.L[1], M1 = d.L[2], P = M1 * M2;
-    PassLaunch A = make(0, c2r ? 1 : 0, 1);
+    PassLaunch A = make(0, c2r ? 1 : 0, 1, PM_COLS_TW);
     A.a.in_col = 1; A.a.in_elem = P; A.a.in_sub = 0;
     A.a.out_col = 1; A.a.out_elem = P; A.a.out_sub = 0;
     A.a.in_cf = 1; A.a.out_cf = 1;
     A.gx = (unsigned)(P / d.C[0]); A.gy = 1; A.gz = p->batch;
-    PassLaunch B = make(1, 1, 1);
+    PassLaunch B = make(1, 1, 1, PM_COLS_TW);
     B.a.in_col = 1; B.a.in_elem = M1; B.a.in_sub = P;
     B.a.out_col = 1; B.a.out_elem = M1; B.a.out_sub = P;
     B.a.in_cf = 1; B.a.out_cf = 1;
     B.gx = (unsigned)(M1 / d.C[1]); B.gy = (unsigned)Q; B.gz = p->batch;
-    PassLaunch Cc = make(2, 1, 2);
+    PassLaunch Cc = make(2, 1, 2, PM_ROWS_TO_COLS);
     Cc.a.in_col = P; Cc.a.in_elem = 1; Cc.a.in_sub = M1;
     Cc.a.out_col = 1; Cc.a.out_elem = Q * M2; Cc.a.out_sub = Q;
     Cc.a.in_cf = 0; Cc.a.out_cf = 1;
@@ -632,7 +666,9 @@ int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t n
   const uint64_t cap_in = p->M * p->batch, cap_out = L.dst == 2 ? p->M * p->batch : p->M * p->batch;
   int rc = check_pass_bounds(L, a, ntrials, q == 0 ? 0 : cap_in, cap_out, q == 0 ? p->M : 0);
   if (rc) return rc;
-  hipLaunchKernelGGL(L.k->fn, dim3(L.gx, L.gy, ntrials), dim3(L.k->threads), L.k->lds, hs, a);
+  const KernelInfo* k = q == 0 ? L.krs : L.k;
+  if (!k) return psx::fail(PSX_ERR_UNSUPPORTED, "exec_fused_pass: no resampling kernel for this pass");
+  hipLaunchKernelGGL(k->fn, dim3(L.gx, L.gy, ntrials), dim3(k->threads), k->lds, hs, a);
   PSX_CHECK_LAUNCH("k_fft_pass(fused)");
   return PSX_OK;
 }

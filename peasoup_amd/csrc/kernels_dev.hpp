@@ -29,6 +29,16 @@ __device__ inline uint64_t resample_index(uint64_t i, uint64_t n, double af) {
   return j < n ? j : n - 1;
 }
 
+// The same index for series shorter than 2^31 samples, in 32-bit integers
+// (identical values: every operand is exactly representable either way).
+__device__ inline uint32_t resample_index32(uint32_t i, uint32_t n, double dn, double af) {
+  const double di = (double)i;
+  double v = fma(di * af, di - dn, di);
+  v = v > 0.0 ? v : 0.0;
+  const uint32_t j = (uint32_t)rint(v);
+  return j < n ? j : n - 1;
+}
+
 // bin_interbin_series_kernel (src/kernels.cu:250-257): the 0.5 is a double
 // literal; halving in float gives the identical correctly rounded result.
 __device__ inline float interbin(float2 x, float2 prev) {
@@ -115,6 +125,37 @@ __device__ inline void harmonic_sums(const float* __restrict__ P, uint64_t i, fl
   if (NF > 4) {
 #pragma unroll
     for (uint64_t j = 1; j < 32; j += 2) val += P[(j * i + 16) >> 5];
+    s[4] = (float)((double)val * 0.17677669529663688110);
+  }
+}
+
+// The same sums with 32-bit bin arithmetic (valid for i < 2^27, where every
+// j*i + 2^(m-1) fits in 32 bits).
+template <int NF>
+__device__ inline void harmonic_sums32(const float* __restrict__ P, uint32_t i, float (&s)[5]) {
+  float val = P[i];
+  if (NF > 0) {
+    val += P[(i + 1) >> 1];
+    s[0] = (float)((double)val * 0.70710678118654752440);
+  }
+  if (NF > 1) {
+    val += P[(3 * i + 2) >> 2];
+    val += P[(i + 2) >> 2];
+    s[1] = val * 0.5f;
+  }
+  if (NF > 2) {
+#pragma unroll
+    for (uint32_t j = 1; j < 8; j += 2) val += P[(j * i + 4) >> 3];
+    s[2] = (float)((double)val * 0.35355339059327376220);
+  }
+  if (NF > 3) {
+#pragma unroll
+    for (uint32_t j = 1; j < 16; j += 2) val += P[(j * i + 8) >> 4];
+    s[3] = val * 0.25f;
+  }
+  if (NF > 4) {
+#pragma unroll
+    for (uint32_t j = 1; j < 32; j += 2) val += P[(j * i + 16) >> 5];
     s[4] = (float)((double)val * 0.17677669529663688110);
   }
 }

@@ -102,3 +102,36 @@ def tune(n, tsamp, d_series, accs, batches=(8, 16, 32), reps=5, log=print):
             if best is None or per_trial < best[0]:
                 best = (per_trial, b, dec, desc.value.decode())
     return {"batch": best[1], "decomp": best[2], "ms_per_trial": best[0], "plan": best[3]}
+
+
+def main(argv=None):
+    """Tune the given lengths on device 0 and write a table (JSON).
+
+    python -m peasoup_amd.tuning --sizes 8388608,4194304 --out tuned.json
+    """
+    import argparse
+    import sys
+    import peasoup_amd as P
+    import peasoup_amd.workloads as W
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sizes", default="8388608")
+    ap.add_argument("--tsamp", type=float, default=76e-6)
+    ap.add_argument("--batches", default="8,16,32")
+    ap.add_argument("--limit", type=int, default=12)
+    ap.add_argument("--out", default=CACHE)
+    a = ap.parse_args(argv)
+    dev = P.device_info(0)["name"]
+    for n in [int(x) for x in a.sizes.split(",")]:
+        w = W.Workload("tune", 1, n, a.tsamp, 0x5EA5AA, 5.0, -100, 100)
+        d = W.generate_device(w)
+        accs = W.accel_list(n, a.tsamp, -100, 100)
+        best = tune(n, a.tsamp, d.ptr, accs, batches=[int(b) for b in a.batches.split(",")],
+                    log=lambda *m: print(*m, file=sys.stderr, flush=True))
+        entry = {"batch": best["batch"], "decomp": best["decomp"], "ms_per_trial": round(best["ms_per_trial"], 5),
+                 "plan": best["plan"]}
+        store(n, dev, entry, a.out)
+        print(json.dumps({"n": n, **entry}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

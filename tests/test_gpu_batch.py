@@ -85,3 +85,27 @@ def test_batched_overflow_is_an_error(psx):
     args = W.make_search_args(w.size, w.tsamp, acc_start=0, acc_end=0, min_snr=-10.0)
     with pytest.raises(psx.PsxError):
         W.batch_search(args, series, w.dm_list(), None, W.make_batch_opts(1, cand_capacity=10000))
+
+
+def test_batched_2pow26_three_pass_matches_worker(psx):
+    """Long-segment length 2^26 (three FFT passes, fused first pass)."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("t", 1, 1 << 26, 64e-6, 41, 5.0, -50, 50, injections=[W.Injection(0, 0.0023, 12.0, 25.0, 0.7)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)[1500:1504]
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(2))
+    assert same(got, ref)
+    assert np.any(got["snr"] > 10)
+
+
+def test_fft_2pow26_vs_exact(psx, orc):
+    n = 1 << 26
+    x = np.random.default_rng(26).standard_normal(n).astype(np.float32)
+    plan = psx.FFTPlan(n, psx.PSX_FFT_R2C)
+    d_x, d_X = psx.DeviceArray.from_numpy(x), psx.DeviceArray(n // 2 + 1, np.complex64)
+    plan.execute(d_x, d_X)
+    got = d_X.numpy()
+    ref = orc.fft_r2c_f64(x.astype(np.float64))
+    err = float(np.max(np.abs(got - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)))
+    assert err < 2e-6, (plan.describe(), err)
