@@ -197,7 +197,14 @@ typedef struct psx_search_args {
 } psx_search_args;
 
 /* CandidatePOD fields (include/data_types/candidates.hpp:10-17) plus the
- * associated-candidate count (Candidate::count_assoc, :44-51). */
+ * associated-candidate count (Candidate::count_assoc, :42-49).
+ *
+ * Candidate lists are written as a forest: each top-level candidate is
+ * followed by the candidates associated with it, in pre-order (the order of
+ * Candidate::collect_candidates, :78-84, which is also the order of the
+ * detections in candidates.peasoup), and nassoc counts the records of a
+ * record's own subtree that follow it.  Top-level candidates are therefore
+ * at i0 = 0, i1 = i0 + 1 + nassoc[i0], ... */
 typedef struct psx_candidate {
   float dm;
   int32_t dm_idx;
@@ -213,8 +220,8 @@ typedef struct psx_candidate {
  * nsamps samples each (row-major, DispersionTrials layout), one call per
  * kernel as the reference does.  accs (naccs values) overrides the
  * AccelerationPlan list when non-NULL.  Writes the acceleration-distilled
- * candidates of every DM in DM order (*count = total; PSX_ERR_OVERFLOW if it
- * exceeds cap).  seconds[0..2] (optional) = prologue, accel loop, distill. */
+ * candidates of every DM in DM order, as a forest with their associated
+ * candidates (*count = records; PSX_ERR_OVERFLOW if it exceeds cap).  seconds[0..2] (optional) = prologue, accel loop, distill. */
 int psx_worker_search(const psx_search_args* args, const float* series, uint64_t nsamps, const float* dm_list,
                       uint32_t ndm, const float* accs, uint32_t naccs, psx_candidate* out, uint32_t cap,
                       uint32_t* count, double* seconds);
@@ -293,12 +300,69 @@ int psx_server_create(const psx_server_config* cfg, psx_server** server);
 int psx_server_submit(psx_server* server, const psx_request* req, uint64_t* id);
 int psx_server_poll(psx_server* server, uint64_t id, psx_request_status* status);
 int psx_server_wait(psx_server* server, uint64_t id, double timeout_s, psx_request_status* status);
-/* Candidates of a completed request (per DM, accel-distilled, DM order);
- * releases the request. */
+/* Candidates of a completed request (per DM, accel-distilled, DM order, as a
+ * forest; status.ncand = records); releases the request. */
 int psx_server_result(psx_server* server, uint64_t id, psx_candidate* out, uint32_t cap, uint32_t* count);
 int psx_server_stats(psx_server* server, psx_server_counters* stats);
 /* Completes queued and running requests, then frees everything. */
 int psx_server_destroy(psx_server* server);
+
+/* ------------------------------------------------------------------ */
+/* Global post-processing and output files                             */
+/* ------------------------------------------------------------------ */
+/* Arguments of the program-level distilling and scoring
+ * (src/pipeline_multi.cu:406-407,547-555).  cfreq, foff and bw are in the
+ * filterbank header's units (MHz), as CandidateScorer receives them. */
+typedef struct psx_final_args {
+  float freq_tol;  /* --freq_tol (DMDistiller, HarmonicDistiller) */
+  float max_harm;  /* --max_harm (HarmonicDistiller) */
+  float tsamp;     /* Filterbank::get_tsamp */
+  float cfreq;     /* Filterbank::get_cfreq */
+  float foff;      /* Filterbank::get_foff */
+  float bw;        /* |foff| * nchans */
+  int32_t limit;   /* --limit: candidates kept after scoring */
+  int32_t reserved;
+} psx_final_args;
+
+/* Candidate fields set by CandidateScorer::score (include/transforms/
+ * scorer.hpp:77-81) and the association counts overview.xml reports. */
+typedef struct psx_cand_score {
+  float ddm_count_ratio;
+  float ddm_snr_ratio;
+  int32_t is_adjacent;
+  int32_t is_physical;
+  int32_t nassoc;   /* Candidate::count_assoc */
+  int32_t ndirect;  /* Candidate::assoc.size() */
+} psx_cand_score;
+
+/* main()'s post-processing of the candidates of every DM trial
+ * (src/pipeline_multi.cu:547-555): DMDistiller(freq_tol, true), then
+ * HarmonicDistiller(freq_tol, max_harm, true, false), then
+ * CandidateScorer(tsamp, cfreq, foff, bw).score_all, then the first `limit`.
+ * in: the concatenated forests of the searches (DM order).  out: the final
+ * candidates as a forest (*count records).  scores (optional, scap entries):
+ * one per final top-level candidate (*nfinal of them).  Host only. */
+int psx_finalize(const psx_candidate* in, uint32_t nin, const psx_final_args* args, psx_candidate* out,
+                 uint32_t cap, uint32_t* count, psx_cand_score* scores, uint32_t scap, uint32_t* nfinal);
+
+/* A folded profile (nints x nbins floats, row-major) for candidates.peasoup. */
+typedef struct psx_fold {
+  int32_t nbins;
+  int32_t nints;
+  const float* data;
+} psx_fold;
+
+/* CandidateFileWriter::write_binary (include/utils/output_stats.hpp:261-298):
+ * per top-level candidate of the forest, an optional "FOLD" record (the four
+ * characters, int32 nbins, int32 nints, nbins*nints float32), then int32
+ * ndets and the ndets = 1 + nassoc CandidatePOD records (24 bytes each) of the
+ * candidate and its associates in pre-order.  Native byte order, as the
+ * reference writes it.  folds (optional): one per top-level candidate
+ * (nbins*nints == 0: no FOLD record).  offsets (optional): each candidate's
+ * byte offset (CandidateFileWriter::byte_mapping).  Both arrays hold
+ * ntop_cap entries; *ntop = top-level candidates written. */
+int psx_write_candidates(const char* path, const psx_candidate* forest, uint32_t n, const psx_fold* folds,
+                         int64_t* offsets, uint32_t ntop_cap, uint32_t* ntop);
 
 #ifdef __cplusplus
 }

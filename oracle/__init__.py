@@ -90,6 +90,9 @@ def lib():
                                         C.c_void_p]),
             "orc_search_accels": (C.c_int, [f32p, C.POINTER(SearchParams), f32p, i32, f32, f32, candp,
                                             i32, i32]),
+            "orc_score": (C.c_int, [candp, i32, f32, f32, f32, f32p, i32p]),
+            "orc_finalize": (C.c_int, [candp, i32, f32, f32, f32, f32, f32, i32, candp, i32, f32p, i32p,
+                                       C.POINTER(C.c_int)]),
             "orc_hardware_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -284,6 +287,47 @@ def dm_distill(cands, tol, keep_related=True):
     return out[:n]
 
 
+SCORE_DTYPE = np.dtype([("ddm_count_ratio", "<f4"), ("ddm_snr_ratio", "<f4"), ("is_adjacent", "<i4"),
+                        ("is_physical", "<i4")])
+
+
+def _scores(ratios, flags, n):
+    out = np.zeros(n, SCORE_DTYPE)
+    out["ddm_count_ratio"] = ratios[0:2 * n:2]
+    out["ddm_snr_ratio"] = ratios[1:2 * n:2]
+    out["is_adjacent"] = flags[0:2 * n:2]
+    out["is_physical"] = flags[1:2 * n:2]
+    return out
+
+
+def score(forest, cfreq, foff, bw):
+    """CandidateScorer over the top-level candidates of a forest."""
+    c = np.ascontiguousarray(forest, dtype=CAND_DTYPE)
+    ratios = np.zeros(2 * max(c.size, 1), np.float32)
+    flags = np.zeros(2 * max(c.size, 1), np.int32)
+    n = lib().orc_score(c, c.size, cfreq, foff, bw, ratios, flags)
+    if n < 0:
+        raise ValueError("malformed candidate forest")
+    return _scores(ratios, flags, n)
+
+
+def finalize(forest, freq_tol, max_harm, cfreq, foff, bw, limit, cap=None):
+    """main()'s post-processing: (final forest, scores of its top-level candidates)."""
+    c = np.ascontiguousarray(forest, dtype=CAND_DTYPE)
+    cap = cap or max(4 * c.size, 1024)
+    out = np.zeros(cap, CAND_DTYPE)
+    ratios = np.zeros(2 * max(c.size, 1), np.float32)
+    flags = np.zeros(2 * max(c.size, 1), np.int32)
+    nfin = C.c_int()
+    n = lib().orc_finalize(c, c.size, freq_tol, max_harm, cfreq, foff, bw, limit, out, cap, ratios, flags,
+                           C.byref(nfin))
+    if n < 0:
+        raise ValueError("malformed candidate forest")
+    if n > cap:
+        raise RuntimeError("orc_finalize: %d records exceed cap %d" % (n, cap))
+    return out[:n], _scores(ratios, flags, nfin.value)
+
+
 def prologue(series, params):
     series = _f32(series)
     tim = np.empty(params.size, np.float32)
@@ -311,17 +355,29 @@ def trial(tim, params, acc, mean, std, raw=False, cap=1 << 20):
     return out[:n]
 
 
-def search_accels(tim, params, accs, mean, std, nthreads=1, cap=1 << 20):
+def top_level(forest):
+    """Top-level records of a candidate forest (nassoc = subtree records after each)."""
+    na = np.asarray(forest["nassoc"], np.int64)
+    idx, i = [], 0
+    while i < len(forest):
+        idx.append(i)
+        i += 1 + int(na[i])
+    return forest[np.asarray(idx, np.int64)]
+
+
+def search_accels(tim, params, accs, mean, std, nthreads=1, cap=1 << 20, forest=False):
     tim = _f32(tim)
     accs = _f32(accs)
     out = np.zeros(cap, CAND_DTYPE)
     n = lib().orc_search_accels(tim, C.byref(params), accs, accs.size, mean, std, out, cap, nthreads)
     if n < 0:
         raise RuntimeError("orc_search_accels failed (%d)" % n)
-    return out[:min(n, cap)]
+    if n > cap:
+        raise RuntimeError("orc_search_accels: %d records exceed cap %d" % (n, cap))
+    return out[:n] if forest else top_level(out[:n])
 
 
-def search_dm(series, params, accs, nthreads=1, cap=1 << 20):
+def search_dm(series, params, accs, nthreads=1, cap=1 << 20, forest=False):
     series = _f32(series)
     accs = _f32(accs)
     out = np.zeros(cap, CAND_DTYPE)
@@ -330,7 +386,9 @@ def search_dm(series, params, accs, nthreads=1, cap=1 << 20):
                             t.ctypes.data)
     if n < 0:
         raise RuntimeError("orc_search_dm failed (%d)" % n)
-    return out[:min(n, cap)], t
+    if n > cap:
+        raise RuntimeError("orc_search_dm: %d records exceed cap %d" % (n, cap))
+    return (out[:n] if forest else top_level(out[:n])), t
 
 
 def hardware_threads():

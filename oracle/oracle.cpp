@@ -769,7 +769,146 @@ int to_flat(const std::vector<Cand>& v, orc_cand* out, int cap) {
   return n;
 }
 
+/* Pre-order forest (Candidate::collect_candidates order, candidates.hpp:78-84)
+ * with each record's subtree size in nassoc.  Returns the record count. */
+void forest_rec(const Cand& c, orc_cand* out, int cap, int& pos) {
+  if (pos < cap) {
+    out[pos].dm = c.dm;
+    out[pos].dm_idx = c.dm_idx;
+    out[pos].acc = c.acc;
+    out[pos].nh = c.nh;
+    out[pos].snr = c.snr;
+    out[pos].freq = c.freq;
+    out[pos].nassoc = c.count_assoc();
+    out[pos]._pad = 0;
+  }
+  ++pos;
+  for (const Cand& a : c.assoc) forest_rec(a, out, cap, pos);
+}
+
+int to_forest(const std::vector<Cand>& v, orc_cand* out, int cap) {
+  int pos = 0;
+  for (const Cand& c : v) forest_rec(c, out, cap, pos);
+  return pos;
+}
+
+/* Inverse of to_forest; -1 if a subtree runs past n. */
+int from_forest_rec(const orc_cand* in, int n, int i, Cand& c) {
+  c.dm = in[i].dm;
+  c.dm_idx = in[i].dm_idx;
+  c.acc = in[i].acc;
+  c.nh = in[i].nh;
+  c.snr = in[i].snr;
+  c.freq = in[i].freq;
+  c.assoc.clear();
+  if (in[i].nassoc < 0) return -1;
+  const long end = (long)i + 1 + in[i].nassoc;
+  if (end > n) return -1;
+  int j = i + 1;
+  while (j < end) {
+    c.assoc.emplace_back();
+    j = from_forest_rec(in, n, j, c.assoc.back());
+    if (j < 0 || j > end) return -1;
+  }
+  return j;
+}
+
+/* CandidateScorer (include/transforms/scorer.hpp:8-87): float members,
+ * double-literal expressions narrowed on assignment, as written there. */
+struct Scores {
+  float ddm_count_ratio, ddm_snr_ratio;
+  bool is_adjacent, is_physical;
+};
+
+struct Scorer {
+  float chan_partial, band_partial;
+  Scorer(float cfreq, float foff, float bw) {
+    float ftop = cfreq + bw / 2.0;     /* scorer.hpp:71 */
+    float fbottom = cfreq - bw / 2.0;  /* :72 */
+    chan_partial = 8300.0 * foff / std::pow((double)cfreq, 3.0);                                   /* :73 */
+    band_partial = 4150.0 * (1.0 / std::pow((double)fbottom, 2.0) - 1.0 / std::pow((double)ftop, 2.0)); /* :74 */
+  }
+  Scores score(const Cand& c) const {
+    Scores r;
+    /* has_physical_period, :25-27 */
+    float tdm = c.dm * chan_partial;
+    r.is_physical = 1.0 / c.freq > tdm;
+    /* has_adjacency, :29-45 */
+    bool adjacent = false, unique = true;
+    for (size_t ii = 0; ii < c.assoc.size(); ii++) {
+      if (c.assoc[ii].dm_idx != c.dm_idx) unique = false;
+      if (c.assoc[ii].dm_idx == c.dm_idx + 1 || c.assoc[ii].dm_idx == c.dm_idx - 1) {
+        adjacent = true;
+        break;
+      }
+    }
+    r.is_adjacent = adjacent || unique;
+    /* delta_dm_ratio, :47-65 */
+    int inside_count = 1, total_count = 1;
+    float inside_snr = c.snr, total_snr = c.snr;
+    float ddm = 1.0 / (c.freq * band_partial);
+    for (size_t ii = 0; ii < c.assoc.size(); ii++) {
+      total_count++;
+      total_snr += c.assoc[ii].snr;
+      if (std::fabs(c.dm - c.assoc[ii].dm) <= ddm) {
+        inside_count++;
+        inside_snr += c.assoc[ii].snr;
+      }
+    }
+    r.ddm_count_ratio = (float)inside_count / total_count;
+    r.ddm_snr_ratio = inside_snr / total_snr;
+    return r;
+  }
+};
+
 }  // namespace
+
+extern "C" int orc_score(const orc_cand* forest, int n, float cfreq, float foff, float bw, float* ratios,
+                         int* flags) {
+  int i = 0, t = 0;
+  Scorer sc(cfreq, foff, bw);
+  while (i < n) {
+    Cand c;
+    int j = from_forest_rec(forest, n, i, c);
+    if (j < 0) return -1;
+    Scores r = sc.score(c);
+    ratios[2 * t] = r.ddm_count_ratio;
+    ratios[2 * t + 1] = r.ddm_snr_ratio;
+    flags[2 * t] = r.is_adjacent;
+    flags[2 * t + 1] = r.is_physical;
+    ++t;
+    i = j;
+  }
+  return t;
+}
+
+extern "C" int orc_finalize(const orc_cand* in, int n, float freq_tol, float max_harm, float cfreq, float foff,
+                            float bw, int limit, orc_cand* out, int cap, float* ratios, int* flags, int* nfinal) {
+  std::vector<Cand> v;
+  int i = 0;
+  while (i < n) {
+    v.emplace_back();
+    i = from_forest_rec(in, n, i, v.back());
+    if (i < 0) return -1;
+  }
+  /* pipeline_multi.cu:406-407,547-548 */
+  v = dm_distill(v, freq_tol, true);
+  v = harmonic_distill(v, freq_tol, max_harm, true, false);
+  /* :550-552 */
+  Scorer sc(cfreq, foff, bw);
+  /* :557-558 */
+  const size_t keep = std::min((size_t)std::max(limit, 0), v.size());
+  v.resize(keep);
+  for (size_t t = 0; t < v.size(); ++t) {
+    Scores r = sc.score(v[t]);
+    ratios[2 * t] = r.ddm_count_ratio;
+    ratios[2 * t + 1] = r.ddm_snr_ratio;
+    flags[2 * t] = r.is_adjacent;
+    flags[2 * t + 1] = r.is_physical;
+  }
+  *nfinal = (int)v.size();
+  return to_forest(v, out, cap);
+}
 
 extern "C" int orc_harmonic_distill(const orc_cand* in, int n, float tol, float max_harm, int keep_related,
                                     int fractional, orc_cand* out) {
@@ -914,7 +1053,7 @@ extern "C" int orc_search_accels(const float* tim, const orc_search_params* p, c
   std::vector<Cand> all;
   for (auto& v : per) all.insert(all.end(), v.begin(), v.end());
   std::vector<Cand> d = accel_distill(all, tobs_of(p), p->freq_tol, true);
-  return to_flat(d, out, cap);
+  return to_forest(d, out, cap);
 }
 
 extern "C" int orc_search_dm(const float* series, uint64_t n_real, const orc_search_params* p, const float* accs,

@@ -27,6 +27,9 @@
  *                             (src/harmonic_sum_test.cpp:21-27)
  *   orc_find_candidates       brute-force threshold scan + clustering cases
  *   orc_*_distill             oracle/_ref build of the reference distiller.hpp
+ *   orc_score / orc_finalize  oracle/_ref build of the reference scorer.hpp
+ *                             and distiller.hpp chained as main() chains them;
+ *                             hand-computed scorer cases
  */
 #pragma once
 #include <stdint.h>
@@ -133,7 +136,10 @@ int orc_trial(const float* tim, const orc_search_params* p, float acc, float mea
 
 /* Whole DM trial (prologue + accel loop + acceleration distill) on nthreads
  * host threads (one accel trial per thread).  timings[0] = prologue s,
- * timings[1] = accel loop s, timings[2] = distill s (may be NULL). */
+ * timings[1] = accel loop s, timings[2] = distill s (may be NULL).
+ * Output (also of orc_search_accels) is a forest: each distilled candidate
+ * followed by its associates in pre-order, nassoc = subtree records after it;
+ * returns the record count. */
 int orc_search_dm(const float* series, uint64_t n_real, const orc_search_params* p,
                   const float* accs, int nacc, orc_cand* out, int cap, int nthreads,
                   double* timings);
@@ -141,6 +147,19 @@ int orc_search_dm(const float* series, uint64_t n_real, const orc_search_params*
 /* Accel-loop only (given a prologue result), used by the CPU baseline. */
 int orc_search_accels(const float* tim, const orc_search_params* p, const float* accs, int nacc,
                       float mean, float std_, orc_cand* out, int cap, int nthreads);
+
+/* CandidateScorer::score (include/transforms/scorer.hpp:25-81) of every
+ * top-level candidate of a forest: ratios[2t..2t+1] = ddm_count_ratio,
+ * ddm_snr_ratio; flags[2t..2t+1] = is_adjacent, is_physical.  Returns the
+ * number of top-level candidates, -1 if the forest is malformed. */
+int orc_score(const orc_cand* forest, int n, float cfreq, float foff, float bw, float* ratios, int* flags);
+
+/* main()'s post-processing, src/pipeline_multi.cu:406-407,547-558: DM distill
+ * (keep related), harmonic distill (keep related, no fractional harmonics),
+ * score, keep the first `limit`.  in/out are forests; returns the output
+ * record count (written up to cap), *nfinal = top-level candidates. */
+int orc_finalize(const orc_cand* in, int n, float freq_tol, float max_harm, float cfreq, float foff, float bw,
+                 int limit, orc_cand* out, int cap, float* ratios, int* flags, int* nfinal);
 
 int orc_hardware_threads(void);
 

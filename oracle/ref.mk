@@ -1,6 +1,6 @@
-# Builds oracle/_ref/libref.so from the reference's own distiller sources
-# (include/transforms/distiller.hpp, include/data_types/candidates.hpp: standard
-# C++ only).  The rest of the reference's search path needs CUDA, cuFFT,
+# Builds oracle/_ref/libref.so from the reference's own distiller and scorer
+# sources (include/transforms/distiller.hpp, include/transforms/scorer.hpp,
+# include/data_types/candidates.hpp: standard C++ only).  The rest of the reference's search path needs CUDA, cuFFT,
 # Thrust or libdedisp and is not buildable here.
 # Usage: make -C oracle -f ref.mk   (no-op when the reference is absent)
 REF ?= /root/reference

@@ -262,9 +262,9 @@ void psx_server::finish(Request* r) {
     std::vector<Candidate> dd = still.distill(acc_cands);
     flat.insert(flat.end(), dd.begin(), dd.end());
   }
-  r->result.resize(flat.size());
+  r->result.resize(psx::search::forest_size(flat));
   uint32_t cnt = 0;
-  psx::search::flatten(flat, r->result.data(), (uint32_t)flat.size(), &cnt);
+  psx::search::flatten(flat, r->result.data(), (uint32_t)r->result.size(), &cnt);
   r->per_trial.clear();
   (void)hipFree(r->d_tims);
   r->d_tims = nullptr;

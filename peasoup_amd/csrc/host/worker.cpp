@@ -102,17 +102,6 @@ void run_worker(const psx_search_args& args, DispersionTrials<float>& trials, co
 namespace psx {
 namespace search {
 
-int flatten(const std::vector<Candidate>& cands, psx_candidate* out, uint32_t cap, uint32_t* count) {
-  const uint32_t n = (uint32_t)cands.size();
-  if (count) *count = n;
-  for (uint32_t i = 0; i < n && i < cap; ++i) {
-    const Candidate& c = cands[i];
-    out[i] = psx_candidate{c.dm, c.dm_idx, c.acc, c.nh, c.snr, c.freq, c.count_assoc(), 0};
-  }
-  if (n > cap) return psx::fail(PSX_ERR_OVERFLOW, "candidate output holds %u of %u", cap, n);
-  return PSX_OK;
-}
-
 int validate_args(const psx_search_args* a) {
   PSX_REQUIRE(a, "null search args");
   PSX_REQUIRE(a->size >= 64 && a->size <= (1ull << 28), "size must be in [64, 2^28]");

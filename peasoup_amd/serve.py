@@ -9,7 +9,7 @@ import ctypes as C
 import numpy as np
 
 import peasoup_amd as P
-from peasoup_amd.workloads import CAND_DTYPE, SearchArgs
+from peasoup_amd.workloads import CAND_DTYPE, SearchArgs, top_level
 
 
 class ServerConfig(C.Structure):
@@ -89,12 +89,15 @@ class Server:
             s["error_text"] = P.lib().psx_last_error_string().decode()
         return s
 
-    def result(self, rid, cap=1 << 20):
+    def result(self, rid, cap=1 << 20, forest=False):
+        """Top-level candidates of a finished request (the whole forest with
+        forest=True); releases the request."""
         out = np.zeros(cap, CAND_DTYPE)
         cnt = C.c_uint32()
         P.check(P.lib().psx_server_result(self.h, rid, out.ctypes.data, cap, C.byref(cnt)))
         self._keep.pop(rid, None)
-        return out[:cnt.value]
+        out = out[:cnt.value]
+        return out if forest else top_level(out)
 
     def stats(self):
         c = Counters()

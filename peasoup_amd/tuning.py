@@ -74,7 +74,7 @@ def candidates(n, limit=12):
     return out[:limit]
 
 
-def tune(n, tsamp, d_series, accs, batches=(8, 16, 32), reps=5, log=print):
+def tune(n, tsamp, d_series, accs, batches=(8, 16, 32), reps=5, limit=12, log=print):
     """Times candidate schedules on one prologued series; returns the best."""
     import peasoup_amd as P
     import peasoup_amd.workloads as W
@@ -85,7 +85,7 @@ def tune(n, tsamp, d_series, accs, batches=(8, 16, 32), reps=5, log=print):
     args = W.make_search_args(n, tsamp)
     accs = np.ascontiguousarray(accs, np.float32)
     best = None
-    for dec in [None] + candidates(n):
+    for dec in [None] + candidates(n, limit):
         for b in batches:
             opts = W.make_batch_opts(b, decomp=dec)
             ms = np.zeros(8)
@@ -125,7 +125,7 @@ def main(argv=None):
         w = W.Workload("tune", 1, n, a.tsamp, 0x5EA5AA, 5.0, -100, 100)
         d = W.generate_device(w)
         accs = W.accel_list(n, a.tsamp, -100, 100)
-        best = tune(n, a.tsamp, d.ptr, accs, batches=[int(b) for b in a.batches.split(",")],
+        best = tune(n, a.tsamp, d.ptr, accs, batches=[int(b) for b in a.batches.split(",")], limit=a.limit,
                     log=lambda *m: print(*m, file=sys.stderr, flush=True))
         entry = {"batch": best["batch"], "decomp": best["decomp"], "ms_per_trial": round(best["ms_per_trial"], 5),
                  "plan": best["plan"]}

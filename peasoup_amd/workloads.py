@@ -168,8 +168,21 @@ P.register_signatures({
 })
 
 
-def worker_search(args, series, dm_list, accs=None, cap=1 << 20):
-    """Plain Worker sequence (one kernel per reference call) on host series."""
+def top_level(forest):
+    """Top-level records of a candidate forest (each record is followed by the
+    nassoc records of its associated candidates, in pre-order)."""
+    na = np.asarray(forest["nassoc"], np.int64)
+    idx = []
+    i, n = 0, len(forest)
+    while i < n:
+        idx.append(i)
+        i += 1 + int(na[i])
+    return forest[np.asarray(idx, np.int64)]
+
+
+def worker_search(args, series, dm_list, accs=None, cap=1 << 20, forest=False):
+    """Plain Worker sequence (one kernel per reference call) on host series.
+    Returns the top-level candidates, or the whole forest with forest=True."""
     series = np.ascontiguousarray(series, np.float32)
     dm_list = np.ascontiguousarray(dm_list, np.float32)
     out = np.zeros(cap, CAND_DTYPE)
@@ -180,12 +193,14 @@ def worker_search(args, series, dm_list, accs=None, cap=1 << 20):
                                       dm_list.size, None if a is None else a.ctypes.data,
                                       0 if a is None else a.size, out.ctypes.data, cap, C.byref(cnt),
                                       secs.ctypes.data))
-    return out[:cnt.value], secs
+    out = out[:cnt.value]
+    return (out if forest else top_level(out)), secs
 
 
-def batch_search(args, series, dm_list, accs=None, opts=None, nsamps=None, cap=1 << 20):
+def batch_search(args, series, dm_list, accs=None, opts=None, nsamps=None, cap=1 << 20, forest=False):
     """Batched fused search.  series: host array [ndm, nsamps] or a DeviceArray
-    holding ndm x nsamps samples (then nsamps is required)."""
+    holding ndm x nsamps samples (then nsamps is required).  Returns the
+    top-level candidates, or the whole forest with forest=True."""
     dm_list = np.ascontiguousarray(dm_list, np.float32)
     on_dev = isinstance(series, P.DeviceArray)
     if on_dev:
@@ -201,7 +216,8 @@ def batch_search(args, series, dm_list, accs=None, opts=None, nsamps=None, cap=1
     P.check(P.lib().psx_batch_search(C.byref(args), C.byref(o), ptr, 1 if on_dev else 0, ns, dm_list.ctypes.data,
                                      dm_list.size, None if a is None else a.ctypes.data, 0 if a is None else a.size,
                                      out.ctypes.data, cap, C.byref(cnt), secs.ctypes.data))
-    return out[:cnt.value], secs
+    out = out[:cnt.value]
+    return (out if forest else top_level(out)), secs
 
 
 def generate_device(w: Workload, trials=None):

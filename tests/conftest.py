@@ -28,3 +28,11 @@ def orc():
     import oracle
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def psx_host():
+    """The product bindings for host-only entry points (no device needed)."""
+    import peasoup_amd as P
+    P.lib()  # raises if the library is not built
+    return P

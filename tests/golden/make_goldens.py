@@ -7,6 +7,12 @@ repository, example_output/overview.xml) and writes, next to this script:
                               (overview.xml:91-10850, 10,758 values)
   overview_search_params.json the search/header parameters that fix that list
                               (overview.xml:15-19,40-62)
+  overview_candidates.npy     the 1,250 final candidates' fields
+                              (overview.xml:10860-29611): period, dm, acc,
+                              nh, snr, scores, nassoc, byte_offset
+  example_candidates.peasoup  the reference's example candidates.peasoup
+                              (example_output/candidates.peasoup, a data
+                              file: 10 FOLD + detection records), copied
 
 Run once in a container that has the reference mounted; the outputs are
 committed so the tests do not need the reference.
@@ -19,6 +25,10 @@ import sys
 import numpy as np
 
 SRC = "/root/reference/example_output/overview.xml"
+CANDS = "/root/reference/example_output/candidates.peasoup"
+CAND_FIELDS = [("period", "<f8"), ("opt_period", "<f8"), ("dm", "<f8"), ("acc", "<f8"), ("nh", "<i4"),
+               ("snr", "<f8"), ("folded_snr", "<f8"), ("is_adjacent", "<i4"), ("is_physical", "<i4"),
+               ("ddm_count_ratio", "<f8"), ("ddm_snr_ratio", "<f8"), ("nassoc", "<i4"), ("byte_offset", "<i8")]
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -55,6 +65,19 @@ def main():
     with open(os.path.join(HERE, "overview_search_params.json"), "w") as f:
         json.dump(params, f, indent=1, sort_keys=True)
     print("wrote %d accelerations" % len(vals), file=sys.stderr)
+
+    block = re.search(r"<candidates>(.*)</candidates>", txt, re.S).group(1)
+    rows = []
+    for m in re.finditer(r"<candidate id='(\d+)'>(.*?)</candidate>", block, re.S):
+        assert int(m.group(1)) == len(rows)
+        body = m.group(2)
+        rows.append(tuple(float(re.search(r"<%s>([^<]*)</%s>" % (k, k), body).group(1)) for k, _ in CAND_FIELDS))
+    np.save(os.path.join(HERE, "overview_candidates.npy"), np.array(rows, dtype=CAND_FIELDS))
+    with open(CANDS, "rb") as f:
+        data = f.read()
+    with open(os.path.join(HERE, "example_candidates.peasoup"), "wb") as f:
+        f.write(data)
+    print("wrote %d candidates, %d bytes of candidates.peasoup" % (len(rows), len(data)), file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -43,6 +43,35 @@ def test_batched_vs_oracle(psx, orc, cfg1):
     assert not compare(got, np.concatenate(ref), w.min_snr)
 
 
+def test_final_candidates_vs_oracle(psx, orc, cfg1):
+    """Series -> searches -> main()'s DM/harmonic distilling, scoring and limit:
+    the batched forests equal the plain Worker's, the library's finalize equals
+    the oracle's on them bit for bit, and the final list matches the oracle's
+    end-to-end computation."""
+    import peasoup_amd.postprocess as pp
+    import peasoup_amd.workloads as W
+    w, series, accs = cfg1
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, forest=True)
+    plain, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs, forest=True)
+    assert same(got, plain) and np.array_equal(got["nassoc"], plain["nassoc"])
+    ref = []
+    for d in range(w.ndm):
+        p = orc.make_params(w.size, np.float32(w.tsamp), dm=float(d), dm_idx=d, nharmonics=w.nharmonics)
+        ref.append(orc.search_dm(series[d], p, accs, nthreads=4, forest=True)[0])
+    ref = np.concatenate(ref)
+    args = pp.make_final_args(freq_tol=1e-4, max_harm=16.0, tsamp=w.tsamp, cfreq=1400.0, foff=0.390625,
+                              nchans=1024, limit=1000)
+    fin, sc = pp.finalize(got, args)
+    ofin, osc = orc.finalize(got.view(orc.CAND_DTYPE), 1e-4, 16.0, 1400.0, 0.390625, args.bw, 1000)
+    assert np.array_equal(fin.view(np.uint8), ofin.view(np.uint8))
+    assert np.array_equal(sc["is_adjacent"], osc["is_adjacent"])
+    assert np.array_equal(sc["ddm_count_ratio"], osc["ddm_count_ratio"])
+    rfin, _ = orc.finalize(ref, 1e-4, 16.0, 1400.0, 0.390625, args.bw, 1000)
+    tops, rtops = W.top_level(fin), orc.top_level(rfin)
+    assert len(tops) > 0 and len(tops) < len(W.top_level(got))
+    assert not compare(tops, rtops, w.min_snr)
+
+
 def test_batched_device_input_and_plan(psx, cfg1):
     import peasoup_amd.workloads as W
     w, series, accs = cfg1
@@ -92,7 +121,9 @@ def test_batched_2pow26_three_pass_matches_worker(psx):
     import peasoup_amd.workloads as W
     w = W.Workload("t", 1, 1 << 26, 64e-6, 41, 5.0, -50, 50, injections=[W.Injection(0, 0.0023, 12.0, 25.0, 0.7)])
     series = W.generate_host(w)
-    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)[1500:1504]
+    full = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    k = int(np.argmin(np.abs(full - 12.0)))  # the trials around the injected acceleration
+    accs = full[k - 2:k + 2]
     ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs)
     got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(2))
     assert same(got, ref)
