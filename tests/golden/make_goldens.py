@@ -13,6 +13,8 @@ repository, example_output/overview.xml) and writes, next to this script:
   example_candidates.peasoup  the reference's example candidates.peasoup
                               (example_output/candidates.peasoup, a data
                               file: 10 FOLD + detection records), copied
+  example_overview.xml.xz     example_output/overview.xml, xz-compressed
+                              (the byte-exact format fixture of overview.xml)
 
 Run once in a container that has the reference mounted; the outputs are
 committed so the tests do not need the reference.
@@ -78,6 +80,9 @@ def main():
     with open(os.path.join(HERE, "example_candidates.peasoup"), "wb") as f:
         f.write(data)
     print("wrote %d candidates, %d bytes of candidates.peasoup" % (len(rows), len(data)), file=sys.stderr)
+    import lzma
+    with open(SRC, "rb") as f, open(os.path.join(HERE, "example_overview.xml.xz"), "wb") as g:
+        g.write(lzma.compress(f.read(), preset=9))
 
 
 if __name__ == "__main__":
