@@ -99,6 +99,12 @@ int psx_median_scrunch5(const float* in, uint64_t count, float* out, psx_stream 
 int psx_linear_stretch(const float* in, uint64_t in_count, float* out, uint64_t out_count, psx_stream s);
 /* device_divide_c_by_f (src/kernels.cu:1040-1061, kernels.h:290-294). */
 int psx_divide_c_by_f(psx_c32* X, const float* f, uint64_t nb, psx_stream s);
+/* Zapper::zap (include/transforms/birdiezapper.hpp:60-71) ->
+ * device_zap_birdies (src/kernels.cu:1063-1096): for each birdie (device
+ * arrays freqs/widths, Hz), bins [floor((f-w)/bin_width), ceil((f+w)/bin_width))
+ * clamped to [0, nb-1) are set to 1+0i. */
+int psx_zap_birdies(psx_c32* X, const float* freqs, const float* widths, uint32_t nbirdies, float bin_width,
+                    uint64_t nb, psx_stream s);
 /* Dereddener::calculate_median (dereddener.hpp:41-62): running median of A
  * into median (nb), using work (>= psx_deredden_work_floats(nb) floats). */
 uint64_t psx_deredden_work_floats(uint64_t nb);
@@ -194,6 +200,9 @@ typedef struct psx_search_args {
   float cfreq;            /* centre frequency, Hz (get_cfreq()*1e6) */
   float ch_bw;            /* channel bandwidth, Hz (get_foff()*1e6) */
   int32_t min_gap;        /* PeakFinder min_gap (30) */
+  uint32_t nbirdies;      /* birdies to zap after dereddening (--zapfile); 0 = none */
+  const float* birdie_freq;   /* host, nbirdies frequencies (Hz); read during the call */
+  const float* birdie_width;  /* host, nbirdies half-widths (Hz) */
 } psx_search_args;
 
 /* CandidatePOD fields (include/data_types/candidates.hpp:10-17) plus the

@@ -30,14 +30,22 @@ class SearchParams(C.Structure):
                 ("dm_idx", C.c_int32), ("min_snr", C.c_float), ("min_freq", C.c_float),
                 ("max_freq", C.c_float), ("nharmonics", C.c_int32), ("freq_tol", C.c_float),
                 ("max_harm", C.c_float), ("min_gap", C.c_int32),
-                ("boundary_5_freq", C.c_float), ("boundary_25_freq", C.c_float)]
+                ("boundary_5_freq", C.c_float), ("boundary_25_freq", C.c_float), ("nbirdies", C.c_int32),
+                ("birdie_freq", C.c_void_p), ("birdie_width", C.c_void_p)]
 
 
 def make_params(size, tsamp, dm=0.0, dm_idx=0, min_snr=7.0, min_freq=0.1, max_freq=1100.0,
                 nharmonics=4, freq_tol=1e-4, max_harm=16.0, min_gap=30,
-                boundary_5_freq=0.05, boundary_25_freq=0.5):
-    return SearchParams(size, tsamp, dm, dm_idx, min_snr, min_freq, max_freq, nharmonics,
-                        freq_tol, max_harm, min_gap, boundary_5_freq, boundary_25_freq)
+                boundary_5_freq=0.05, boundary_25_freq=0.5, birdies=None):
+    p = SearchParams(size, tsamp, dm, dm_idx, min_snr, min_freq, max_freq, nharmonics,
+                     freq_tol, max_harm, min_gap, boundary_5_freq, boundary_25_freq, 0, None, None)
+    if birdies is not None and len(birdies):
+        b = np.asarray(birdies, np.float32).reshape(-1, 2)
+        p._birdies = (np.ascontiguousarray(b[:, 0]), np.ascontiguousarray(b[:, 1]))
+        p.nbirdies = len(b)
+        p.birdie_freq = p._birdies[0].ctypes.data
+        p.birdie_width = p._birdies[1].ctypes.data
+    return p
 
 
 _lib = None
@@ -71,6 +79,7 @@ def lib():
             "orc_linear_stretch": (None, [f32p, u64, f32p, u64]),
             "orc_calculate_median": (None, [f32p, u64, C.c_double, f32, f32, f32p]),
             "orc_divide_c_by_f": (None, [f32p, f32p, u64]),
+            "orc_zap_birdies": (None, [f32p, u64, f32, f32p, f32p, i32]),
             "orc_stats": (None, [f32p, u64, u64, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                  C.POINTER(C.c_float)]),
             "orc_normalise": (None, [f32p, f32, f32, u64]),
@@ -353,6 +362,15 @@ def trial(tim, params, acc, mean, std, raw=False, cap=1 << 20):
         k = min(rc.value, cap)
         return out[:n], (rn[:k], ri[:k], rs[:k])
     return out[:n]
+
+
+def zap_birdies(X, bin_width, birdies):
+    """X: complex64 spectrum (modified copy returned); birdies: (n, 2) freq, half width."""
+    X = np.ascontiguousarray(X, np.complex64).copy()
+    b = np.asarray(birdies, np.float32).reshape(-1, 2)
+    lib().orc_zap_birdies(X.view(np.float32), X.size, bin_width, np.ascontiguousarray(b[:, 0]),
+                          np.ascontiguousarray(b[:, 1]), len(b))
+    return X
 
 
 def top_level(forest):

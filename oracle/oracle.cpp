@@ -481,6 +481,27 @@ extern "C" void orc_divide_c_by_f(float* X, const float* f, uint64_t nb) {
   }
 }
 
+/* zap_birdies_kernel, kernels.cu:1063-1084: for each birdie, bins from
+ * __float2int_rd((f-w)/bw) up to (excluding) __float2int_ru((f+w)/bw), with
+ * the low end raised to 0, birdies starting at or past nb skipped and the
+ * high end lowered to nb-1, are set to 1+0i. */
+extern "C" void orc_zap_birdies(float* X, uint64_t nb, float bin_width, const float* freqs, const float* widths,
+                                int n) {
+  for (int b = 0; b < n; ++b) {
+    float freq = freqs[b];
+    float width = widths[b];
+    long low_bin = (long)std::floor((freq - width) / bin_width);
+    long high_bin = (long)std::ceil((freq + width) / bin_width);
+    if (low_bin < 0) low_bin = 0;
+    if (low_bin >= (long)nb) continue;
+    if (high_bin >= (long)nb) high_bin = (long)nb - 1;
+    for (long ii = low_bin; ii < high_bin; ii++) {
+      X[2 * ii] = 1.0f;
+      X[2 * ii + 1] = 0.0f;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Statistics: stats.hpp:6-40, kernels.cu:427-521                            */
 /* ------------------------------------------------------------------------- */
@@ -1015,6 +1036,8 @@ extern "C" int orc_prologue(const float* series, uint64_t n_real, const orc_sear
   orc_form_amplitude(X.data(), A.data(), nb);
   orc_calculate_median(A.data(), nb, (double)bin_width_of(p), p->boundary_5_freq, p->boundary_25_freq, med.data());
   orc_divide_c_by_f(X.data(), med.data(), nb);
+  /* pipeline_multi.cu:222-226 */
+  if (p->nbirdies > 0) orc_zap_birdies(X.data(), nb, bin_width_of(p), p->birdie_freq, p->birdie_width, p->nbirdies);
   orc_form_interbin(X.data(), P.data(), nb);
   orc_stats(P.data(), nb, 0, mean, rms, std_);
   std::vector<cd> z, work;

@@ -27,6 +27,7 @@
  *                             (src/harmonic_sum_test.cpp:21-27)
  *   orc_find_candidates       brute-force threshold scan + clustering cases
  *   orc_*_distill             oracle/_ref build of the reference distiller.hpp
+ *   orc_zap_birdies           hand-computed bin ranges (exact binary fractions)
  *   orc_score / orc_finalize  oracle/_ref build of the reference scorer.hpp
  *                             and distiller.hpp chained as main() chains them;
  *                             hand-computed scorer cases
@@ -65,6 +66,9 @@ typedef struct {
   int32_t min_gap;        /* PeakFinder min_gap (30) */
   float boundary_5_freq;  /* Dereddener zone boundaries (0.05, 0.5) */
   float boundary_25_freq;
+  int32_t nbirdies;       /* zap list (Zapper): 0 = none */
+  const float* birdie_freq;
+  const float* birdie_width;
 } orc_search_params;
 
 /* include/utils/utils.hpp:143-233 */
@@ -97,6 +101,8 @@ void orc_linear_stretch(const float* in, uint64_t in_count, float* out, uint64_t
 void orc_calculate_median(const float* A, uint64_t nb, double bin_width,
                           float b5, float b25, float* median);
 void orc_divide_c_by_f(float* X, const float* f, uint64_t nb);
+/* src/kernels.cu:1063-1084 (Zapper::zap, birdiezapper.hpp:60-71) */
+void orc_zap_birdies(float* X, uint64_t nb, float bin_width, const float* freqs, const float* widths, int n);
 
 /* include/utils/stats.hpp:6-40, src/kernels.cu:427-521 */
 void orc_stats(const float* p, uint64_t n, uint64_t first, float* mean, float* rms, float* std_);

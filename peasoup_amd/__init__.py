@@ -74,6 +74,7 @@ def lib():
         "psx_median_scrunch5": (i32, [vp, u64, vp, vp]),
         "psx_linear_stretch": (i32, [vp, u64, vp, u64, vp]),
         "psx_divide_c_by_f": (i32, [vp, vp, u64, vp]),
+        "psx_zap_birdies": (i32, [vp, vp, vp, u32, f32, u64, vp]),
         "psx_deredden_work_floats": (u64, [u64]),
         "psx_deredden_median": (i32, [vp, u64, f64, f32, f32, vp, vp, vp]),
         "psx_reduce_work_bytes": (u64, []),

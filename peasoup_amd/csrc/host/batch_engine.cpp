@@ -119,7 +119,8 @@ Engine::~Engine() {
   (void)hipFree(red_work);
 }
 
-void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std) {
+void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std,
+                      const Zapper* zap) {
   // pipeline_multi.cu:170-238
   const uint64_t ncopy = std::min<uint64_t>(n_real, N);
   if (d_series != d_tim) hip_ok(hipMemcpyAsync(d_tim, d_series, ncopy * sizeof(float), hipMemcpyDeviceToDevice, s),
@@ -132,6 +133,7 @@ void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, floa
   psx_ok(psx_form_amplitude(X, A, nb, s), "form");
   psx_ok(psx_deredden_median(A, nb, (double)bin_width, 0.05f, 0.5f, med, med_work, s), "calculate_median");
   psx_ok(psx_divide_c_by_f(X, med, nb, s), "deredden");
+  if (zap) zap->zap_on(X, bin_width, nb, s);
   psx_ok(psx_form_interbin(X, A, nb, s), "form_interpolated");
   float m = 0, r = 0;
   psx_ok(psx_mean_rms(A, nb, 0, &m, &r, red_work, s), "stats");

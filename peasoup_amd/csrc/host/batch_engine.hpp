@@ -68,7 +68,9 @@ class Engine {
 
   // Worker prologue (pipeline_multi.cu:170-238) for one DM with the plain
   // kernels: d_series (n_real samples) -> d_tim (N samples), mean/std.
-  void prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std);
+  // zap (optional) removes birdies after dereddening (pipeline_multi.cu:222-226).
+  void prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std,
+                const Zapper* zap = nullptr);
 
   // Average duration of each launch of one batch of n trials, timed with HIP
   // events on the engine stream over `reps` repetitions (after one warm-up).

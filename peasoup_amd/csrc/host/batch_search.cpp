@@ -110,8 +110,10 @@ extern "C" int psx_batch_profile(const psx_search_args* args, const psx_batch_op
     PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
     PSX_CHECK_HIP(hipMalloc(&d_tim, sizeof(float) * args->size));
+    std::unique_ptr<Zapper> zap;
+    if (args->nbirdies) zap.reset(new Zapper(args->birdie_freq, args->birdie_width, args->nbirdies, s));
     float mean, sd;
-    eng->prologue(d_series, nsamps, d_tim, &mean, &sd);
+    eng->prologue(d_series, nsamps, d_tim, &mean, &sd, zap.get());
     const uint32_t n = std::min(naccs, eng->batch());
     std::vector<TrialWork> tw(n);
     for (uint32_t t = 0; t < n; ++t) tw[t] = TrialWork{d_tim, accs[t], mean, sd, 0.0f, 0};
@@ -163,6 +165,8 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
     AccelerationPlan plan(args->acc_start, args->acc_end, args->acc_tol, args->acc_pulse_width * 1e-6, N,
                           args->tsamp, args->cfreq, args->ch_bw);
     if (!series_on_device) PSX_CHECK_HIP(hipMalloc(&d_stage, sizeof(float) * nreal));
+    std::unique_ptr<Zapper> zap;
+    if (args->nbirdies) zap.reset(new Zapper(args->birdie_freq, args->birdie_width, args->nbirdies, s));
     for (uint32_t d = 0; d < ndm; ++d) {
       float* tim = d_tims + (uint64_t)d * N;
       const float* src = series + (uint64_t)d * nreal;
@@ -170,7 +174,7 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
         PSX_CHECK_HIP(hipMemcpyAsync(d_stage, src, sizeof(float) * nreal, hipMemcpyHostToDevice, s));
         src = d_stage;
       }
-      eng->prologue(src, nreal, tim, &means[d], &stds[d]);
+      eng->prologue(src, nreal, tim, &means[d], &stds[d], zap.get());
       tims[d] = tim;
       idx[d] = (int)d;
       if (accs)

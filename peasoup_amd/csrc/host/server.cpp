@@ -83,6 +83,8 @@ struct Request {
   psx_request spec;
   std::vector<float> dm_list;
   std::vector<float> accs_override;
+  std::vector<float> birdie_freq, birdie_width;  // the request's zap list (copied at submit)
+  std::unique_ptr<Zapper> zap;                   // on the device while the request is admitted
   int state = ST_WAITING;
   int error = PSX_OK;
   std::string error_text;
@@ -190,6 +192,7 @@ void psx_server::fail_request(Request* r, int code, const std::string& msg) {
   r->error = code;
   r->error_text = msg;
   r->t_done = now_s();
+  r->zap.reset();
   if (r->d_tims) {
     (void)hipFree(r->d_tims);
     r->d_tims = nullptr;
@@ -227,12 +230,14 @@ bool psx_server::admit_one(Request* r) {
   // every prologue and acceleration list is prepared and checked before any
   // trial of the request is queued, so a request that fails here leaves no
   // work behind
+  if (!r->birdie_freq.empty())
+    r->zap.reset(new Zapper(r->birdie_freq.data(), r->birdie_width.data(), r->birdie_freq.size(), es->stream));
   for (uint32_t d = 0; d < ndm; ++d) {
     float* tim = r->d_tims + (uint64_t)d * N;
     if (hipMemcpyAsync(tim, r->spec.series + (uint64_t)d * r->spec.nsamps, ncopy * sizeof(float), kind,
                        es->stream) != hipSuccess)
       throw std::runtime_error("series upload failed");
-    es->eng->prologue(tim, r->spec.nsamps, tim, &r->means[d], &r->stds[d]);
+    es->eng->prologue(tim, r->spec.nsamps, tim, &r->means[d], &r->stds[d], r->zap.get());
     if (!r->accs_override.empty())
       r->acc_lists[d] = r->accs_override;
     else
@@ -266,6 +271,7 @@ void psx_server::finish(Request* r) {
   uint32_t cnt = 0;
   psx::search::flatten(flat, r->result.data(), (uint32_t)r->result.size(), &cnt);
   r->per_trial.clear();
+  r->zap.reset();
   (void)hipFree(r->d_tims);
   r->d_tims = nullptr;
   used -= r->bytes;
@@ -373,6 +379,11 @@ extern "C" int psx_server_submit(psx_server* s, const psx_request* req, uint64_t
   r->spec = *req;
   r->dm_list.assign(req->dm_list, req->dm_list + req->ndm);
   if (req->accs) r->accs_override.assign(req->accs, req->accs + req->naccs);
+  if (req->args.nbirdies) {
+    r->birdie_freq.assign(req->args.birdie_freq, req->args.birdie_freq + req->args.nbirdies);
+    r->birdie_width.assign(req->args.birdie_width, req->args.birdie_width + req->args.nbirdies);
+  }
+  r->spec.args.birdie_freq = r->spec.args.birdie_width = nullptr;  // the caller's arrays are not kept
   r->bytes = request_bytes(*req);
   r->t_submit = now_s();
   std::lock_guard<std::mutex> lk(s->mu);

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <vector>
 
 #include "peasoup/candidates.hpp"
@@ -49,6 +50,9 @@ void run_worker(const psx_search_args& args, DispersionTrials<float>& trials, co
   AccelerationDistiller acc_still(tobs, args.freq_tol, true);
   AccelerationPlan acc_plan(args.acc_start, args.acc_end, args.acc_tol, args.acc_pulse_width * 1e-6, size,
                             args.tsamp, args.cfreq, args.ch_bw);
+  // pipeline_multi.cu:136-141: the zap list is optional
+  std::unique_ptr<Zapper> bzap;
+  if (args.nbirdies) bzap.reset(new Zapper(args.birdie_freq, args.birdie_width, args.nbirdies));
   void* stats_work = nullptr;
   peasoup::check(psx_malloc(&stats_work, psx_reduce_work_bytes()), "Worker");
   float mean, std, rms;
@@ -67,6 +71,7 @@ void run_worker(const psx_search_args& args, DispersionTrials<float>& trials, co
       former.form(d_fseries, pspec);
       rednoise.calculate_median(pspec);
       rednoise.deredden(d_fseries);
+      if (bzap) bzap->zap(d_fseries);  // pipeline_multi.cu:222-226
       former.form_interpolated(d_fseries, pspec);
       stats::stats_with(pspec.get_data(), size / 2 + 1, &mean, &rms, &std, stats_work);
       c2rfft.execute(d_fseries.get_data(), d_tim.get_data());
@@ -108,6 +113,7 @@ int validate_args(const psx_search_args* a) {
   PSX_REQUIRE(a->tsamp > 0, "tsamp must be > 0");
   PSX_REQUIRE(a->nharmonics >= 1 && a->nharmonics <= 5, "nharmonics must be 1..5");
   PSX_REQUIRE(a->min_gap >= 1, "min_gap must be >= 1");
+  PSX_REQUIRE(!a->nbirdies || (a->birdie_freq && a->birdie_width), "birdie arrays missing");
   return PSX_OK;
 }
 

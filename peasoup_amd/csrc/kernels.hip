@@ -143,6 +143,22 @@ __global__ void __launch_bounds__(kBlock) k_divide(float2* __restrict__ X, const
   }
 }
 
+// zap_birdies_kernel (src/kernels.cu:1063-1084): bins
+// [floor((f-w)/bw), ceil((f+w)/bw)) of each birdie, clamped to [0, nb-1),
+// become 1+0i.  One block per birdie; the bin range is computed in float as
+// the reference does and clamped before any integer conversion.
+__global__ void __launch_bounds__(kBlock) k_zap(float2* __restrict__ X, const float* __restrict__ freqs,
+                                                const float* __restrict__ widths, float bin_width, uint64_t nb) {
+  const float f = freqs[blockIdx.x];
+  const float w = widths[blockIdx.x];
+  const double lo_f = (double)floorf((f - w) / bin_width);
+  const double hi_f = (double)ceilf((f + w) / bin_width);
+  if (!(lo_f < (double)nb)) return;  // also rejects NaN
+  const uint64_t lo = lo_f > 0.0 ? (uint64_t)lo_f : 0;
+  const uint64_t hi = hi_f >= (double)(nb - 1) ? nb - 1 : (hi_f > 0.0 ? (uint64_t)hi_f : 0);
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) X[i] = make_float2(1.0f, 0.0f);
+}
+
 // --------------------------------------------------------------- reductions
 // Deterministic two-level sum of x and of float(x*x) in double (GPU_mean /
 // GPU_rms, src/kernels.cu:433-461, accumulate float terms in float).
@@ -402,6 +418,16 @@ extern "C" int psx_divide_c_by_f(psx_c32* X, const float* f, uint64_t nb, psx_st
   if (!nb) return psx::ok();
   k_divide<<<psx::grid_for(nb, kBlock), kBlock, 0, as_stream(s)>>>((float2*)X, f, nb);
   PSX_CHECK_LAUNCH("k_divide");
+  return psx::ok();
+}
+
+extern "C" int psx_zap_birdies(psx_c32* X, const float* freqs, const float* widths, uint32_t nbirdies,
+                               float bin_width, uint64_t nb, psx_stream s) {
+  if (!nbirdies || !nb) return psx::ok();
+  PSX_REQUIRE(X && freqs && widths && bin_width > 0, "psx_zap_birdies: bad arguments");
+  PSX_REQUIRE(nbirdies <= 0x7FFFFFFFu, "psx_zap_birdies: too many birdies");
+  k_zap<<<nbirdies, kBlock, 0, as_stream(s)>>>((float2*)X, freqs, widths, bin_width, nb);
+  PSX_CHECK_LAUNCH("k_zap");
   return psx::ok();
 }
 

@@ -4,6 +4,7 @@
 //   CuFFTerR2C / C2R      include/transforms/ffter.hpp:7-110
 //   SpectrumFormer        include/transforms/spectrumformer.hpp:6-24
 //   Dereddener            include/transforms/dereddener.hpp:10-68
+//   Zapper                include/transforms/birdiezapper.hpp:11-73
 //   HarmonicFolder        include/transforms/harmonicfolder.hpp:8-45
 //   PeakFinder            include/transforms/peakfinder.hpp:11-95
 //   stats::*              include/utils/stats.hpp:6-43
@@ -14,6 +15,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "peasoup/candidates.hpp"
@@ -133,6 +138,73 @@ class Dereddener {
     peasoup::check(psx_divide_c_by_f(spectrum.get_data(), median, spectrum.get_nbins(), stream), "deredden");
   }
   float* get_median() { return median; }
+};
+
+// ------------------------------------------------------ birdie zapper
+// A zap list is lines of "<frequency Hz> <half width Hz>"; blank lines are
+// skipped (birdiezapper.hpp:35-57).  A line with one field is an error here
+// (the reference reads past its split result there).
+class Zapper {
+ private:
+  std::vector<float> birdies;
+  std::vector<float> widths;
+  float* d_birdies = nullptr;
+  float* d_widths = nullptr;
+  psx_stream stream;
+
+  void upload() {
+    psx_free(d_birdies);
+    psx_free(d_widths);
+    d_birdies = d_widths = nullptr;
+    if (birdies.empty()) return;
+    Utils::device_malloc(&d_birdies, birdies.size());
+    Utils::device_malloc(&d_widths, widths.size());
+    Utils::h2dcpy(d_birdies, birdies.data(), birdies.size(), stream);
+    Utils::h2dcpy(d_widths, widths.data(), widths.size(), stream);
+  }
+
+ public:
+  explicit Zapper(const std::string& zaplist, psx_stream s = nullptr) : stream(s) { append_from_file(zaplist); }
+  Zapper(const float* freqs, const float* half_widths, std::size_t n, psx_stream s = nullptr)
+      : birdies(freqs, freqs + n), widths(half_widths, half_widths + n), stream(s) {
+    upload();
+  }
+  Zapper(const Zapper&) = delete;
+  Zapper& operator=(const Zapper&) = delete;
+  ~Zapper() {
+    psx_free(d_birdies);
+    psx_free(d_widths);
+  }
+
+  void append_from_file(const std::string& zaplist) {
+    std::ifstream infile(zaplist.c_str());
+    if (!infile.good()) peasoup::throw_error("Zapper: cannot open " + zaplist);
+    std::string line;
+    while (std::getline(infile, line)) {
+      std::istringstream fields(line);
+      std::string f, w;
+      if (!(fields >> f)) continue;
+      if (!(fields >> w)) peasoup::throw_error("Zapper: line without a width in " + zaplist + ": " + line);
+      birdies.push_back((float)std::atof(f.c_str()));
+      widths.push_back((float)std::atof(w.c_str()));
+    }
+    upload();
+  }
+
+  void zap(DeviceFourierSeries<psx_c32>& fseries) {
+    zap(fseries.get_data(), fseries.get_bin_width(), fseries.get_nbins());
+  }
+  void zap(psx_c32* fseries, float bin_width, unsigned int nbins) {
+    peasoup::check(psx_zap_birdies(fseries, d_birdies, d_widths, (uint32_t)birdies.size(), bin_width, nbins, stream),
+                   "Zapper::zap");
+  }
+  void zap_on(psx_c32* fseries, float bin_width, uint64_t nbins, psx_stream s) const {
+    peasoup::check(psx_zap_birdies(fseries, d_birdies, d_widths, (uint32_t)birdies.size(), bin_width, nbins, s),
+                   "Zapper::zap");
+  }
+  std::size_t size() const { return birdies.size(); }
+  const std::vector<float>& get_birdies() const { return birdies; }
+  const std::vector<float>& get_widths() const { return widths; }
 };
 
 // ------------------------------------------------------ harmonic folder

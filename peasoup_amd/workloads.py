@@ -58,7 +58,8 @@ class SearchArgs(C.Structure):
                 ("max_freq", C.c_float), ("nharmonics", C.c_uint32), ("freq_tol", C.c_float),
                 ("max_harm", C.c_float), ("acc_start", C.c_float), ("acc_end", C.c_float),
                 ("acc_tol", C.c_float), ("acc_pulse_width", C.c_float), ("cdm", C.c_float),
-                ("cfreq", C.c_float), ("ch_bw", C.c_float), ("min_gap", C.c_int32)]
+                ("cfreq", C.c_float), ("ch_bw", C.c_float), ("min_gap", C.c_int32), ("nbirdies", C.c_uint32),
+                ("birdie_freq", C.c_void_p), ("birdie_width", C.c_void_p)]
 
 
 CAND_DTYPE = np.dtype([("dm", "<f4"), ("dm_idx", "<i4"), ("acc", "<f4"), ("nh", "<i4"), ("snr", "<f4"),
@@ -67,9 +68,32 @@ CAND_DTYPE = np.dtype([("dm", "<f4"), ("dm_idx", "<i4"), ("acc", "<f4"), ("nh", 
 
 def make_search_args(size, tsamp, acc_start=-50.0, acc_end=50.0, acc_tol=1.1, acc_pulse_width=64.0,
                      nharmonics=4, min_snr=7.0, min_freq=0.1, max_freq=1100.0, freq_tol=1e-4, max_harm=16.0,
-                     cdm=0.0, cfreq=1.4e9, ch_bw=1e6, min_gap=30):
-    return SearchArgs(size, tsamp, min_snr, min_freq, max_freq, nharmonics, freq_tol, max_harm, acc_start,
-                      acc_end, acc_tol, acc_pulse_width, cdm, cfreq, ch_bw, min_gap)
+                     cdm=0.0, cfreq=1.4e9, ch_bw=1e6, min_gap=30, birdies=None):
+    """birdies: optional (n, 2) frequencies and half-widths in Hz (a zap list)."""
+    a = SearchArgs(size, tsamp, min_snr, min_freq, max_freq, nharmonics, freq_tol, max_harm, acc_start,
+                   acc_end, acc_tol, acc_pulse_width, cdm, cfreq, ch_bw, min_gap, 0, None, None)
+    if birdies is not None and len(birdies):
+        b = np.asarray(birdies, np.float32).reshape(-1, 2)
+        a._birdies = (np.ascontiguousarray(b[:, 0]), np.ascontiguousarray(b[:, 1]))  # kept alive with a
+        a.nbirdies = len(b)
+        a.birdie_freq = a._birdies[0].ctypes.data
+        a.birdie_width = a._birdies[1].ctypes.data
+    return a
+
+
+def read_zaplist(path):
+    """Zap list lines "<frequency> <half width>" (Hz); blank lines skipped
+    (include/transforms/birdiezapper.hpp:35-57)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if not parts:
+                continue
+            if len(parts) < 2:
+                raise ValueError("zap list line without a width: %r" % line)
+            rows.append((float(parts[0]), float(parts[1])))
+    return np.array(rows, np.float32).reshape(-1, 2)
 
 
 def accel_list(size, tsamp, acc_lo, acc_hi, tol=1.1, pulse_width_us=64.0):

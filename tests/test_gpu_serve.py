@@ -90,3 +90,24 @@ def test_failing_request_is_isolated(psx, reqs):
         assert srv.wait(good2)["state"] == "done"
         assert srv.wait(badid)["state"] == "failed"
         assert same(srv.result(good1), srv.result(good2))
+
+
+def test_request_with_zap_list(psx, reqs):
+    """A request's zap list is copied at submit and applied in its prologues;
+    requests with and without one share an engine."""
+    import peasoup_amd.workloads as W
+    from peasoup_amd.serve import Server
+    w, series = reqs[0]
+    zl = [(50.0, 0.1), (100.0, 0.15), (1.0 / 5.3e-3, 0.2)]
+    za = W.make_search_args(w.size, w.tsamp, acc_start=w.acc_lo, acc_end=w.acc_hi, birdies=zl)
+    direct, _ = W.batch_search(za, series, w.dm_list(), None, W.make_batch_opts(8))
+    plain, _ = W.batch_search(w.search_args(), series, w.dm_list(), None, W.make_batch_opts(8))
+    with Server(64 << 30, batch=8) as srv:
+        i1 = srv.submit(W.make_search_args(w.size, w.tsamp, acc_start=w.acc_lo, acc_end=w.acc_hi, birdies=zl),
+                        series, w.dm_list())
+        i2 = srv.submit(w.search_args(), series, w.dm_list())
+        s1, s2 = srv.wait(i1), srv.wait(i2)
+        assert s1["state"] == "done" and s2["state"] == "done", (s1, s2)
+        got1, got2 = srv.result(i1), srv.result(i2)
+    assert same(got1, direct) and same(got2, plain)
+    assert not same(direct, plain)

@@ -192,6 +192,25 @@ def test_divide_c_by_f(psx, orc):
     assert np.array_equal(d_X.numpy(), orc.divide_c_by_f(X, f))
 
 
+DEFAULT_ZAPLIST = [(50.0, 0.1), (100.0, 0.15), (150.0, 0.15), (200.0, 0.15), (250.0, 0.15)]  # misc/default_zaplist.txt
+
+
+@pytest.mark.parametrize("nb,bw", [((1 << 16) + 1, np.float32(1.0 / ((1 << 17) * np.float32(64e-6)))),
+                                   ((1 << 22) + 1, np.float32(1.0 / ((1 << 23) * np.float32(76e-6)))),
+                                   (2001, np.float32(0.25))])
+def test_zap_birdies(psx, orc, nb, bw):
+    X = rand_spectrum(nb)
+    top = float(bw) * (nb - 1)
+    birdies = DEFAULT_ZAPLIST + [(0.01, 0.5), (top - 0.3 * float(bw), 2 * float(bw)), (top + 5.0, 1.0),
+                                 (float(bw) * 1000, 0.0), (float(bw) * 1000.5, 0.0)]
+    b = np.asarray(birdies, np.float32)
+    d_X, d_f, d_w = dev(psx, X), dev(psx, np.ascontiguousarray(b[:, 0])), dev(psx, np.ascontiguousarray(b[:, 1]))
+    psx.check(psx.lib().psx_zap_birdies(d_X.ptr, d_f.ptr, d_w.ptr, len(b), bw, nb, None))
+    ref = orc.zap_birdies(X, bw, birdies)
+    assert np.array_equal(d_X.numpy(), ref)
+    assert np.count_nonzero(ref != X) > 0
+
+
 # ------------------------------------------------------------------ stats
 @pytest.mark.parametrize("n,first", [(100, 0), ((1 << 22) + 1, 0), (1000003, 17)])
 def test_mean_rms(psx, orc, n, first):
