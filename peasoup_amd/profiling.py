@@ -1,0 +1,156 @@
+"""Reduce rocprofv3 CSV output to the summaries kept under profiles/.
+
+Two inputs:
+
+* a kernel trace with statistics (``rocprofv3 --kernel-trace --stats
+  --output-format csv``) -> per-kernel calls / average / total duration;
+* two counter runs of ``bench.py --profile-run`` (``--pmc FETCH_SIZE``, then
+  ``--pmc WRITE_SIZE``) -> HBM bytes per dispatch of each fused kernel,
+  calibrated by the run's copy kernel (2^28 floats read and 2^28 written, so
+  1 GiB each way): factor = counter reading of the copy / 1 GiB, and a
+  kernel's traffic = its reading / factor.  The per-trial traffic of the
+  batched stages is written to profiles/traffic.json, which bench.py reports
+  as roofline.traffic.
+
+Usage:
+  python -m peasoup_amd.profiling stats <dir>            # kernel stats table
+  python -m peasoup_amd.profiling traffic <fetch_dir> <write_dir> --n 8388608 --trials 16 [--out traffic.json]
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+import sys
+
+COPY_BYTES = float(1 << 30)
+COPY_KERNEL = "k_convert"
+
+
+def _rows(path_glob):
+    rows = []
+    for p in sorted(glob.glob(path_glob, recursive=True)):
+        with open(p, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def _col(row, *names):
+    for n in names:
+        if n in row:
+            return row[n]
+    raise KeyError("none of %s in %s" % (names, list(row)))
+
+
+def short_name(kernel):
+    """Kernel name without argument lists, keeping template arguments."""
+    k = re.sub(r"\(.*\)$", "", kernel.strip())
+    return k.replace("psx::", "").replace("(anonymous namespace)::", "")
+
+
+def kernel_stats(d):
+    """[(name, calls, avg_us, total_ms, percent)] from a --stats run."""
+    out = []
+    for r in _rows(os.path.join(d, "**", "*kernel_stats.csv")):
+        out.append((short_name(_col(r, "Name", "KernelName", "Kernel_Name")), int(_col(r, "Calls")),
+                    float(_col(r, "AverageNs")) / 1e3, float(_col(r, "TotalDurationNs")) / 1e6,
+                    float(_col(r, "Percentage"))))
+    out.sort(key=lambda x: -x[3])
+    return out
+
+
+def stage_of(name):
+    """Batched-stage label of a fused kernel name (or None)."""
+    if "k_fft_pass<" in name:
+        return "fft_pass0_resample" if re.search(r",\s*true>\s*$", name) else "fft_pass"
+    if "k_spec_post" in name:
+        return "spec_post"
+    if "k_harm_peaks" in name:
+        return "harm_peaks"
+    return None
+
+
+def counter_by_kernel(d, counter):
+    """{kernel: [(grid, value_bytes, dispatch_id), ...]} (counter in KiB per dispatch)."""
+    res = {}
+    for r in _rows(os.path.join(d, "**", "*counter_collection.csv")):
+        if _col(r, "Counter_Name") != counter:
+            continue
+        name = short_name(_col(r, "Kernel_Name"))
+        grid = int(_col(r, "Grid_Size")) if "Grid_Size" in r else 0
+        res.setdefault(name, []).append((grid, float(_col(r, "Counter_Value")) * 1024.0,
+                                         int(_col(r, "Dispatch_Id"))))
+    return res
+
+
+def traffic(fetch_dir, write_dir, trials):
+    """Per-stage HBM bytes per trial (fetch + write, copy-calibrated) of the
+    batched launches (the largest grid of each kernel)."""
+    out = {"calibration": {}}
+    per = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        by = counter_by_kernel(d, counter)
+        copy = [v for k, vals in by.items() if COPY_KERNEL in k for (_, v, _) in vals]
+        if not copy:
+            raise RuntimeError("no %s dispatch of the copy kernel in %s" % (counter, d))
+        factor = (sum(copy) / len(copy)) / COPY_BYTES
+        out["calibration"][counter] = {"copy_reading_bytes": sum(copy) / len(copy), "copy_bytes": COPY_BYTES,
+                                       "factor": factor}
+        fft_rank = []
+        for k, vals in by.items():
+            st = stage_of(k)
+            if st is None:
+                continue
+            gmax = max(g for g, _, _ in vals)
+            big = [(v, i) for g, v, i in vals if g == gmax]
+            avg = sum(v for v, _ in big) / len(big) / factor
+            if st == "fft_pass":
+                fft_rank.append((min(i for _, i in big), k, avg))
+                continue
+            per.setdefault(st, {"kernel": k})[counter] = avg
+        for q, (_, k, avg) in enumerate(sorted(fft_rank), start=1):
+            per.setdefault("fft_pass%d" % q, {"kernel": k})[counter] = avg
+    for st, e in per.items():
+        e["bytes_per_launch"] = e.get("FETCH_SIZE", 0.0) + e.get("WRITE_SIZE", 0.0)
+        e["bytes_per_trial"] = e["bytes_per_launch"] / trials
+    out["stages"] = per
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s1 = sub.add_parser("stats")
+    s1.add_argument("dir")
+    s1.add_argument("--top", type=int, default=25)
+    s2 = sub.add_parser("traffic")
+    s2.add_argument("fetch_dir")
+    s2.add_argument("write_dir")
+    s2.add_argument("--n", type=int, required=True)
+    s2.add_argument("--trials", type=int, required=True)
+    s2.add_argument("--out")
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        print("%-100s %8s %12s %12s %7s" % ("kernel", "calls", "avg_us", "total_ms", "pct"))
+        for name, calls, avg, tot, pct in kernel_stats(a.dir)[:a.top]:
+            print("%-100s %8d %12.2f %12.3f %7.2f" % (name[:100], calls, avg, tot, pct))
+        return
+    t = traffic(a.fetch_dir, a.write_dir, a.trials)
+    print(json.dumps(t, indent=1))
+    if a.out:
+        table = {}
+        if os.path.exists(a.out):
+            with open(a.out) as f:
+                table = json.load(f)
+        table[str(a.n)] = {st: {"bytes_per_trial": round(e["bytes_per_trial"]), "kernel": e["kernel"],
+                                "fetch_bytes_per_launch": round(e.get("FETCH_SIZE", 0.0)),
+                                "write_bytes_per_launch": round(e.get("WRITE_SIZE", 0.0))}
+                           for st, e in t["stages"].items()}
+        table[str(a.n)]["calibration"] = t["calibration"]
+        with open(a.out, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
