@@ -373,6 +373,59 @@ typedef struct psx_fold {
 int psx_write_candidates(const char* path, const psx_candidate* forest, uint32_t n, const psx_fold* folds,
                          int64_t* offsets, uint32_t ntop_cap, uint32_t* ntop);
 
+/* ------------------------------------------------------------------ */
+/* Upstream of the search: SIGPROC filterbanks and dedispersion        */
+/* ------------------------------------------------------------------ */
+/* SIGPROC header values (include/data_types/header.hpp:171-215). */
+typedef struct psx_sigproc_header {
+  char source_name[80];
+  char rawdatafile[80];
+  double az_start, za_start, src_raj, src_dej, tstart, tsamp, period, fch1, foff, refdm;
+  int32_t nchans, telescope_id, machine_id, data_type, ibeam, nbeams, nbits, barycentric, pulsarcentric, nbins,
+      nsamples, nifs, npuls;
+  uint8_t signed_data;
+  uint8_t reserved[3];
+  uint32_t size; /* header bytes */
+} psx_sigproc_header;
+
+/* read_header (include/data_types/header.hpp:339-403): keyword/value pairs
+ * between HEADER_START and HEADER_END; nsamples from the file size when the
+ * header has none.  PSX_ERR_ARG if the file is not a SIGPROC filterbank. */
+int psx_sigproc_read_header(const char* path, psx_sigproc_header* header);
+/* SigprocFilterbank::load_gulp (include/data_types/filterbank.hpp:313-333):
+ * the packed bytes of samples [start, start + nsamps) into out (host,
+ * nsamps * nchans * nbits / 8 bytes); *nread = whole samples read. */
+int psx_sigproc_read_samples(const char* path, const psx_sigproc_header* header, uint64_t start, uint64_t nsamps,
+                             uint8_t* out, uint64_t* nread);
+/* Filterbank::get_cfreq (filterbank.hpp:231-237), float arithmetic. */
+float psx_sigproc_cfreq(const psx_sigproc_header* header);
+
+/* Dedisperser::generate_dm_list (include/transforms/dedisperser.hpp:62-70)
+ * -> libdedisp dedisp_generate_dm_list: DMs from dm_start while below dm_end,
+ * each step keeping the smearing within tol of the pulse width (us).
+ * tsamp, f0 (fch1), df (foff) in the float precision the plan holds them. */
+int psx_dedisp_dm_list(float dm_start, float dm_end, float tsamp, float pulse_width_us, float f0, float df,
+                       uint32_t nchans, float tol, float* out, uint32_t cap, uint32_t* count);
+/* libdedisp's per-channel delay in samples per unit DM:
+ * 4.15e3/tsamp * (1/(f0 + c df)^2 - 1/f0^2). */
+int psx_dedisp_delay_table(uint32_t nchans, float tsamp, float f0, float df, float* table);
+/* Samples lost at the end of a dedispersed series: the delay of the last
+ * channel at the largest DM of the list, rounded to nearest. */
+uint64_t psx_dedisp_max_delay(const float* dm_list, uint32_t ndm, const float* table, uint32_t nchans);
+/* Device work bytes of psx_dedisperse. */
+uint64_t psx_dedisp_work_bytes(uint64_t nsamps_in, uint32_t nchans, uint32_t ndm);
+/* Dedisperser::dedisperse (dedisperser.hpp:105-186) -> libdedisp
+ * dedisp_execute, direct algorithm, float output: for each DM of dm_list
+ * (host) and t < nsamps_out = nsamps_in - max_delay,
+ *   out[dm * out_stride + t] = sum over unmasked channels c of
+ *       v(t + rn(dm * table[c]), c) / ((2^nbits - 1) * nchans)
+ * where v is the unpacked nbits sample (nbits 1, 2, 4 or 8, lowest bits
+ * first within a byte).  d_in: packed samples on the device; killmask (host,
+ * may be NULL): 0 drops a channel. */
+int psx_dedisperse(const uint8_t* d_in, uint64_t nsamps_in, uint32_t nchans, uint32_t nbits, const float* dm_list,
+                   uint32_t ndm, const int32_t* killmask, float tsamp, float f0, float df, float* d_out,
+                   uint64_t out_stride, void* d_work, psx_stream s);
+
 #ifdef __cplusplus
 }
 #endif

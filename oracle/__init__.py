@@ -102,6 +102,10 @@ def lib():
             "orc_score": (C.c_int, [candp, i32, f32, f32, f32, f32p, i32p]),
             "orc_finalize": (C.c_int, [candp, i32, f32, f32, f32, f32, f32, i32, candp, i32, f32p, i32p,
                                        C.POINTER(C.c_int)]),
+            "orc_dm_list": (C.c_int, [f32, f32, f32, f32, f32, f32, i32, f32, f32p, i32]),
+            "orc_delay_table": (None, [i32, f32, f32, f32, f32p]),
+            "orc_dedisperse": (C.c_long, [np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"), C.c_long, i32,
+                                          i32, f32p, i32, C.c_void_p, f32, f32, f32, f32p]),
             "orc_hardware_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -411,3 +415,34 @@ def search_dm(series, params, accs, nthreads=1, cap=1 << 20, forest=False):
 
 def hardware_threads():
     return lib().orc_hardware_threads()
+
+
+def dm_list(dm_start, dm_end, tsamp, pulse_width_us, f0, df, nchans, tol):
+    """libdedisp's DM list (dedisp_generate_dm_list), float32."""
+    n = lib().orc_dm_list(dm_start, dm_end, tsamp, pulse_width_us, f0, df, nchans, tol, np.zeros(1, np.float32), 0)
+    if n < 0:
+        raise RuntimeError("DM list does not converge")
+    out = np.zeros(n, np.float32)
+    lib().orc_dm_list(dm_start, dm_end, tsamp, pulse_width_us, f0, df, nchans, tol, out, n)
+    return out
+
+
+def delay_table(nchans, tsamp, f0, df):
+    t = np.zeros(nchans, np.float32)
+    lib().orc_delay_table(nchans, tsamp, f0, df, t)
+    return t
+
+
+def dedisperse(packed, nsamps, nchans, nbits, dms, tsamp, f0, df, killmask=None):
+    """Direct dedispersion of packed samples: [ndm, nout] float32."""
+    packed = np.ascontiguousarray(packed, np.uint8)
+    dms = _f32(dms)
+    km = None if killmask is None else np.ascontiguousarray(killmask, np.int32)
+    table = delay_table(nchans, tsamp, f0, df)
+    nout = nsamps - int(np.rint(dms[-1] * table[-1]))
+    out = np.zeros((dms.size, max(nout, 1)), np.float32)
+    n = lib().orc_dedisperse(packed, nsamps, nchans, nbits, dms, dms.size, None if km is None else km.ctypes.data,
+                             tsamp, f0, df, out)
+    if n < 0:
+        raise ValueError("series shorter than the dispersion delay")
+    return out[:, :n]

@@ -1098,3 +1098,74 @@ extern "C" int orc_search_dm(const float* series, uint64_t n_real, const orc_sea
 }
 
 extern "C" int orc_hardware_threads(void) { return (int)std::thread::hardware_concurrency(); }
+
+/* ------------------------------------------------------------------------- */
+/* Dedispersion (libdedisp, called through include/transforms/dedisperser.hpp) */
+/* ------------------------------------------------------------------------- */
+/* libdedisp is not part of the reference tree; these restate its published
+ * host-side DM-list recurrence and delay table and its direct dedispersion
+ * kernel with float output.  The plan holds tsamp, f0 and df as floats. */
+extern "C" int orc_dm_list(float dm_start, float dm_end, float dt, float ti, float f0, float df, int nchans, float tol,
+                           float* out, int cap) {
+  /* the smearing of one DM step may grow the pulse width by at most tol;
+   * times in microseconds, the centre frequency in GHz */
+  double dt_us = (double)dt * 1e6;
+  double f = ((double)f0 + ((double)(nchans / 2) - 0.5) * (double)df) * 1e-3;
+  double tol2 = (double)tol * (double)tol;
+  double a = 8.3 * (double)df / (f * f * f);
+  double a2 = a * a;
+  double b2 = a2 * (double)((double)nchans * (double)nchans / 16.0);
+  double c = (dt_us * dt_us + (double)ti * (double)ti) * (tol2 - 1.0);
+  std::vector<float> v;
+  v.push_back(dm_start);
+  while (v.back() < dm_end) {
+    double prev = v.back();
+    double prev2 = prev * prev;
+    double k = c + tol2 * a2 * prev2;
+    double dm = (b2 * prev + std::sqrt(-a2 * b2 * prev2 + (a2 + b2) * k)) / (a2 + b2);
+    v.push_back((float)dm);
+    if (v.size() > (1u << 24)) return -1;
+  }
+  for (size_t i = 0; i < v.size() && (int)i < cap; ++i) out[i] = v[i];
+  return (int)v.size();
+}
+
+extern "C" void orc_delay_table(int nchans, float dt, float f0, float df, float* table) {
+  for (int c = 0; c < nchans; ++c) {
+    float a = 1.f / (f0 + c * df);
+    float b = 1.f / f0;
+    table[c] = 4.15e3 / dt * (a * a - b * b);
+  }
+}
+
+/* Direct dedispersion of packed time-major samples (nbits per sample, lowest
+ * bits first within a byte): out[d*nout + t] = sum over unmasked channels of
+ * the sample at t + rn(dm_d * table[c]), divided by (2^nbits - 1) * nchans;
+ * nout = nsamps - rn(dm_last * table[nchans-1]).  Returns nout (or -1). */
+extern "C" long orc_dedisperse(const uint8_t* in, long nsamps, int nchans, int nbits, const float* dms, int ndm,
+                               const int* killmask, float dt, float f0, float df, float* out) {
+  std::vector<float> table(nchans);
+  orc_delay_table(nchans, dt, f0, df, table.data());
+  long max_delay = (long)std::nearbyint(dms[ndm - 1] * table[nchans - 1]);
+  long nout = nsamps - max_delay;
+  if (nout <= 0) return -1;
+  long bytes_per_sample = (long)nchans * nbits / 8;
+  unsigned mask = (1u << nbits) - 1u;
+  float in_range = (float)mask;
+  for (int d = 0; d < ndm; ++d) {
+    std::vector<long> delay(nchans);
+    for (int c = 0; c < nchans; ++c) delay[c] = (long)std::nearbyint(dms[d] * table[c]);
+    for (long t = 0; t < nout; ++t) {
+      unsigned sum = 0;
+      for (int c = 0; c < nchans; ++c) {
+        if (killmask && !killmask[c]) continue;
+        long ts = t + delay[c];
+        long bit = (long)c * nbits;
+        unsigned byte = in[ts * bytes_per_sample + bit / 8];
+        sum += (byte >> (bit % 8)) & mask;
+      }
+      out[(long)d * nout + t] = (float)sum * 1.0f / (in_range * nchans);
+    }
+  }
+  return nout;
+}

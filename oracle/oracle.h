@@ -28,6 +28,11 @@
  *   orc_find_candidates       brute-force threshold scan + clustering cases
  *   orc_*_distill             oracle/_ref build of the reference distiller.hpp
  *   orc_zap_birdies           hand-computed bin ranges (exact binary fractions)
+ *   orc_dm_list               golden: the DM trials of the reference's example
+ *                             candidates.peasoup (tutorial.fil, 59 values)
+ *   orc_delay_table           closed form (cold-plasma delay in samples)
+ *   orc_dedisperse            known answers: a dispersed pulse lands in one
+ *                             sample at its DM; hand-packed sub-byte samples
  *   orc_score / orc_finalize  oracle/_ref build of the reference scorer.hpp
  *                             and distiller.hpp chained as main() chains them;
  *                             hand-computed scorer cases
@@ -166,6 +171,16 @@ int orc_score(const orc_cand* forest, int n, float cfreq, float foff, float bw, 
  * record count (written up to cap), *nfinal = top-level candidates. */
 int orc_finalize(const orc_cand* in, int n, float freq_tol, float max_harm, float cfreq, float foff, float bw,
                  int limit, orc_cand* out, int cap, float* ratios, int* flags, int* nfinal);
+
+/* libdedisp (through include/transforms/dedisperser.hpp:62-186):
+ * dedisp_generate_dm_list recurrence (returns the count, writes up to cap),
+ * the per-channel delay table, and direct dedispersion with float output
+ * (returns the output length nsamps - max delay, or -1). */
+int orc_dm_list(float dm_start, float dm_end, float dt, float ti, float f0, float df, int nchans, float tol,
+                float* out, int cap);
+void orc_delay_table(int nchans, float dt, float f0, float df, float* table);
+long orc_dedisperse(const uint8_t* in, long nsamps, int nchans, int nbits, const float* dms, int ndm,
+                    const int* killmask, float dt, float f0, float df, float* out);
 
 int orc_hardware_threads(void);
 

@@ -51,6 +51,9 @@ __device__ inline float2 shfl_f2(float2 v, int src) {
   return make_float2(__shfl(v.x, src), __shfl(v.y, src));
 }
 
+// Each thread takes Q pairs per iteration (stride kB apart, so every wave's
+// lanes stay on consecutive bins) and issues all 2Q loads before any math.
+template <int Q>
 __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, float* __restrict__ P, uint64_t M,
                                                   int64_t pstride, RootTable rt, const TrialDesc* __restrict__ descs) {
   const uint32_t tr = blockIdx.y;
@@ -60,29 +63,40 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
   const float std_n = descs[tr].std_n;
   const uint32_t m = (uint32_t)M, half = m / 2;
   const int lane = (int)(threadIdx.x & 63u);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base <= half; base += stride) {
-    const uint32_t k = base + threadIdx.x;
-    const bool valid = k <= half;
-    float2 xl = make_float2(0.f, 0.f), xu = make_float2(0.f, 0.f);
-    if (valid) {
-      const float2 zk = z[k];
-      const float2 zm = z[k == 0 ? 0 : m - k];
-      const float2 w = root(rt, (uint64_t)k);
-      xl = untangle_one(zk, zm, w);
-      // W_n^(M-k) = -conj(W_n^k); the midpoint is its own mirror and keeps the
-      // lower formula, as the plain pass writes it
-      xu = (k == half) ? xl : untangle_one(zm, zk, make_float2(-w.x, w.y));
-    }
-    float2 xl_prev = shfl_f2(xl, lane > 0 ? lane - 1 : 0);
-    float2 xu_next = shfl_f2(xu, lane < 63 ? lane + 1 : 63);
-    if (valid) {
-      if (lane == 0) xl_prev = k == 0 ? make_float2(0.f, 0.f) : pair_half(z, m, k - 1, rt, false);
-      if (k < half) {
-        if (lane == 63) xu_next = pair_half(z, m, k + 1, rt, true);
-        p[m - k] = normalised(xu, xu_next, mean_n, std_n);
+  const uint32_t stride = gridDim.x * kB * Q;
+  for (uint32_t base = blockIdx.x * kB * Q; base <= half; base += stride) {
+    float2 zk[Q], zm[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint32_t k = base + (uint32_t)q * kB + threadIdx.x;
+      zk[q] = zm[q] = make_float2(0.f, 0.f);
+      if (k <= half) {
+        zk[q] = z[k];
+        zm[q] = z[k == 0 ? 0 : m - k];
       }
-      p[k] = normalised(xl, xl_prev, mean_n, std_n);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint32_t k = base + (uint32_t)q * kB + threadIdx.x;
+      const bool valid = k <= half;
+      float2 xl = make_float2(0.f, 0.f), xu = make_float2(0.f, 0.f);
+      if (valid) {
+        const float2 w = root(rt, (uint64_t)k);
+        xl = untangle_one(zk[q], zm[q], w);
+        // W_n^(M-k) = -conj(W_n^k); the midpoint is its own mirror and keeps
+        // the lower formula, as the plain pass writes it
+        xu = (k == half) ? xl : untangle_one(zm[q], zk[q], make_float2(-w.x, w.y));
+      }
+      float2 xl_prev = shfl_f2(xl, lane > 0 ? lane - 1 : 0);
+      float2 xu_next = shfl_f2(xu, lane < 63 ? lane + 1 : 63);
+      if (valid) {
+        if (lane == 0) xl_prev = k == 0 ? make_float2(0.f, 0.f) : pair_half(z, m, k - 1, rt, false);
+        if (k < half) {
+          if (lane == 63) xu_next = pair_half(z, m, k + 1, rt, true);
+          p[m - k] = normalised(xu, xu_next, mean_n, std_n);
+        }
+        p[k] = normalised(xl, xl_prev, mean_n, std_n);
+      }
     }
   }
 }
@@ -151,8 +165,9 @@ __global__ void k_zero_u32(uint32_t* p) {
 int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, const RootTable& rt,
                      const TrialDesc* d_descs, uint32_t ntr, hipStream_t s) {
   if (M < 8 || M > 0x7FFFFFFFull) return psx::fail(PSX_ERR_ARG, "spec_post: unsupported length");
-  const unsigned gx = psx::grid_for(M / 2 + 1, kB, 2048);
-  k_spec_post<<<dim3(gx, ntr), kB, 0, s>>>(Z, P, M, pstride, rt, d_descs);
+  constexpr int Q = 4;
+  const unsigned gx = psx::grid_for(M / 2 + 1, kB * Q, 2048);
+  k_spec_post<Q><<<dim3(gx, ntr), kB, 0, s>>>(Z, P, M, pstride, rt, d_descs);
   PSX_CHECK_LAUNCH("k_spec_post");
   return PSX_OK;
 }

@@ -15,6 +15,10 @@ repository, example_output/overview.xml) and writes, next to this script:
                               file: 10 FOLD + detection records), copied
   example_overview.xml.xz     example_output/overview.xml, xz-compressed
                               (the byte-exact format fixture of overview.xml)
+  tutorial.fil                example_data/tutorial.fil, copied: the SIGPROC
+                              filterbank (64 channels, 2 bits, 320 us; a
+                              250 ms pulsar at DM 30) the example
+                              candidates.peasoup was searched from
 
 Run once in a container that has the reference mounted; the outputs are
 committed so the tests do not need the reference.
@@ -80,6 +84,9 @@ def main():
     with open(os.path.join(HERE, "example_candidates.peasoup"), "wb") as f:
         f.write(data)
     print("wrote %d candidates, %d bytes of candidates.peasoup" % (len(rows), len(data)), file=sys.stderr)
+    with open("/root/reference/example_data/tutorial.fil", "rb") as f, \
+            open(os.path.join(HERE, "tutorial.fil"), "wb") as g:
+        g.write(f.read())
     import lzma
     with open(SRC, "rb") as f, open(os.path.join(HERE, "example_overview.xml.xz"), "wb") as g:
         g.write(lzma.compress(f.read(), preset=9))
