@@ -84,11 +84,12 @@ def cpu_baseline(O, W, w, series0, accs, threads, target_s=20.0):
                       (t_pro, n, len(accs), t_acc, len(accs))}
 
 
-def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16, log=log):
+def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16, dms=(2, 8), log=log):
     """Bounded Poisson stream of segment-search requests through the server.
 
-    Each request is one beam: DM count uniform in {2..8} (the serving
-    configuration's {16..256} scaled by 1/32 so the stream runs in seconds),
+    Each request is one beam: DM count uniform in dms (default {2..8}: the
+    serving configuration's {16..256} scaled by 1/32 so the stream runs in
+    seconds; --serve-dms 16,256 runs the configuration's own range),
     fft size uniform in {2^20..2^24}, tsamp 64 us, N(0,1) + red noise with 0-2
     injected pulsars, accel range +-50 or +-200 m/s^2; admitted against an
     HBM budget; latency = completion - arrival on the library clock."""
@@ -99,7 +100,7 @@ def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16,
     for k in range(nreq):
         t += rng.exponential(1.0 / rate)
         n = int(1 << rng.integers(20, 25))
-        ndm = int(rng.integers(2, 9))
+        ndm = int(rng.integers(dms[0], dms[1] + 1))
         acc = 50.0 if rng.random() < 0.5 else 200.0
         w = W.Workload("req%d" % k, ndm, n, 64e-6, seed + k + 1, 5.0, -acc, acc)
         for _ in range(int(rng.integers(0, 3))):
@@ -132,8 +133,8 @@ def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16,
            "p99_latency_s": round(float(np.percentile(lat, 99)), 4), "max_latency_s": round(float(lat.max()), 4),
            "trials": int(ntrials), "trials_per_s": round(ntrials / (last - first), 1),
            "wall_s": round(wall, 2), "engines": stats["engines"], "hbm_budget_bytes": int(budget),
-           "stream": "Poisson %.1f req/s; DMs U{2..8}; fft size U{2^20..2^24}; tsamp 64 us; accel +-50 or "
-                     "+-200 m/s^2; red noise + 0-2 pulsars" % rate}
+           "stream": "Poisson %.1f req/s; DMs U{%d..%d}; fft size U{2^20..2^24}; tsamp 64 us; accel +-50 or "
+                     "+-200 m/s^2; red noise + 0-2 pulsars" % (rate, dms[0], dms[1])}
     log("serving", res)
     return res
 
@@ -175,6 +176,30 @@ def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, log=log):
     return res
 
 
+def long_segment(P, W, ndm=2, batch=16, log=log):
+    """Long-segment configuration (32 DM trials x 2^26 samples at 64 us, red
+    noise, 4 binary-like pulsars, accel -50..+50 m/s^2: 3,093 trials per DM),
+    sampled on its first ndm DM trials (the injected trials among them keep
+    their pulsars); one warm-up DM first."""
+    w = W.config("long32x2^26")
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    w.ndm = ndm
+    w.injections = [i for i in w.injections if i.trial < ndm]
+    d = W.generate_device(w)
+    P.synchronize()
+    opts = W.make_batch_opts(batch)
+    W.batch_search(w.search_args(), d, w.dm_list()[:1], accs[:batch], opts, nsamps=w.size)
+    t0 = time.perf_counter()
+    c, s = W.batch_search(w.search_args(), d, w.dm_list(), accs, opts, nsamps=w.size)
+    dt = time.perf_counter() - t0
+    n = ndm * accs.size
+    res = {"trials": n, "seconds": round(dt, 3), "trials_per_s": round(n / dt, 1), "candidates": int(c.size),
+           "sample": "%d of the 32 DM trials x 2^26 samples (67,108,864), %d accelerations each, batch %d" %
+                     (ndm, accs.size, batch)}
+    log("long segment", res)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +214,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--no-long", action="store_true", help="skip the 2^26 long-segment sample")
+    ap.add_argument("--serve-dms", default="2,8", help="DM count range of serving requests (lo,hi)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3: a few fused batches of the workload plus a 1 GiB "
                          "copy kernel (calibrates FETCH_SIZE/WRITE_SIZE), no timing line")
@@ -307,12 +334,15 @@ def main():
         result["cpu_baseline"] = cb
         result["speedup_vs_cpu"] = round(value / cb["value"], 1)
         log("cpu baseline", cb)
-    if not a.no_uneven or not a.no_serve:
+    if not a.no_uneven or not a.no_serve or not a.no_long:
         del d_series
+    if not a.no_long:
+        result["long_segment"] = long_segment(P, W)
     if not a.no_uneven:
         result["uneven_batched"] = uneven_batched(P, W, batch=batch)
     if not a.no_serve:
-        result["serving"] = serve_stream(P, W, nreq=a.serve_requests, batch=batch)
+        lo, hi = (int(x) for x in a.serve_dms.split(","))
+        result["serving"] = serve_stream(P, W, nreq=a.serve_requests, batch=batch, dms=(lo, hi))
     print(json.dumps(result), flush=True)
 
 

@@ -71,7 +71,16 @@ def candidates(n, limit=12):
             for c1 in TILES[m1]:
                 out.append([(m2, c0), (m1, c1)])
     out.sort(key=lambda d: (max(d[0][0], d[1][0]), -d[0][1] - d[1][1]))
-    return out[:limit]
+    out = out[:limit]
+    # three passes of short transforms with 16-column tiles: 128-byte
+    # segments everywhere, for one more pass of traffic
+    short = [64, 128, 256, 512]
+    for q in short:
+        for m2 in short:
+            m1 = m // (q * m2)
+            if q * m2 * m1 == m and m1 in short:
+                out.append([(q, 16), (m2, 16), (m1, 16)])
+    return out
 
 
 def tune(n, tsamp, d_series, accs, batches=(8, 16, 32), reps=5, limit=12, log=print):
