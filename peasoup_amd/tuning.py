@@ -56,14 +56,17 @@ def traffic(n, kernel, trials):
 
 POW2_LENS = [32, 64, 128, 256, 512, 1024, 2048, 4096]
 TILES = {32: [16], 64: [16, 8], 128: [16, 8], 256: [16, 8, 4], 512: [16, 8, 4], 1024: [16, 8, 4, 2],
-         2048: [8, 4, 2], 4096: [4, 2]}
+         2048: [8, 4, 2], 4096: [4, 2],
+         48: [16], 96: [16, 8], 192: [16, 8], 384: [16, 8, 4], 768: [16, 8, 4], 1536: [8, 4], 3072: [4]}
+LENS = sorted(TILES)
 
 
 def candidates(n, limit=12):
-    """Two-pass decompositions (M2 strided, M1 contiguous) of a power-of-two n/2."""
+    """Two-pass decompositions (M2 strided, M1 contiguous) of n/2 (2^k or
+    3*2^k), then three-pass ones of short 16-column passes."""
     m = n // 2
     out = []
-    for m1 in POW2_LENS:
+    for m1 in LENS:
         m2 = m // m1
         if m1 * m2 != m or m2 not in TILES:
             continue
@@ -74,7 +77,7 @@ def candidates(n, limit=12):
     out = out[:limit]
     # three passes of short transforms with 16-column tiles: 128-byte
     # segments everywhere, for one more pass of traffic
-    short = [64, 128, 256, 512]
+    short = [48, 64, 96, 128, 192, 256, 384, 512]
     for q in short:
         for m2 in short:
             m1 = m // (q * m2)

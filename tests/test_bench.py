@@ -1,0 +1,31 @@
+"""bench.py contract on the CPU: it refuses more than one GPU with a message,
+and its algorithmic byte accounting is the per-trial model (first FFT pass
+reads the series through the resampler and writes n/2 complex; every other
+pass reads and writes n/2 complex; the spectrum pass reads n/2 complex and
+writes n/2+1 floats; the harmonic pass reads n/2+1 floats)."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_refuses_several_gpus():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
+
+
+def test_stage_bytes_model():
+    sys.path.insert(0, ROOT)
+    import bench
+    n = 1 << 23
+    m, nb = n // 2, n // 2 + 1
+    two = bench.stage_bytes(n, 2)
+    assert two == [4 * n + 8 * m, 16 * m, 8 * m + 4 * nb, 4 * nb]
+    three = bench.stage_bytes(n, 3)
+    assert three[:3] == [4 * n + 8 * m, 16 * m, 16 * m] and three[3:] == two[2:]
+    # the fused trial moves 24n bytes with two FFT passes (about 28n with three)
+    assert abs(sum(two) - 24 * n) <= 16
