@@ -42,7 +42,7 @@ struct ModeTraits {
 };
 
 template <int S, int L, int C, int MODE>
-__device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds, const PassArgs& a, float2* out,
+__device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, const PassArgs& a, float2* out,
                                  unsigned col0, int t) {
   constexpr int P = points_per_thread(L);
   constexpr int NT = C * L / P;
@@ -71,23 +71,56 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2* lds,
     constexpr int R2 = radix_at(L, S + 1);
     constexpr int NB2 = P / R2;
     constexpr bool cf2 = (S + 1 == NSTG - 1) ? ModeTraits<MODE>::out_cf : false;
-    if (S > 0) __syncthreads();
+    if constexpr (!split_planes(MODE)) {
+      float2* lds2 = reinterpret_cast<float2*>(lds);
+      if (S > 0) __syncthreads();
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      int c, j;
-      bf_coords<L, C, R, NT>(b, t, cf, c, j);
-      const int k = j % Ns;
-      const int idxD = (j / Ns) * Ns * R + k;
+      for (int b = 0; b < NB; ++b) {
+        int c, j;
+        bf_coords<L, C, R, NT>(b, t, cf, c, j);
+        const int k = j % Ns;
+        const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lds_index<L, C>(c, idxD + r * Ns)] = v[b * R + r];
+        for (int r = 0; r < R; ++r) lds2[lds_index<L, C>(c, idxD + r * Ns)] = v[b * R + r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB2; ++b) {
+        int c, j;
+        bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds2[lds_index<L, C>(c, j + r * (L / R2))];
+      }
+    } else {
+    // real plane, then imaginary plane: each point's component is written
+    // before it is overwritten by the read of the same component
+#pragma unroll
+    for (int plane = 0; plane < 2; ++plane) {
+      if (S > 0 || plane > 0) __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        int c, j;
+        bf_coords<L, C, R, NT>(b, t, cf, c, j);
+        const int k = j % Ns;
+        const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lds_index<L, C>(c, idxD + r * Ns)] = plane ? v[b * R + r].y : v[b * R + r].x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < NB2; ++b) {
+        int c, j;
+        bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+          const float x = lds[lds_index<L, C>(c, j + r * (L / R2))];
+          if (plane)
+            v[b * R2 + r].y = x;
+          else
+            v[b * R2 + r].x = x;
+        }
+      }
     }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < NB2; ++b) {
-      int c, j;
-      bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
-#pragma unroll
-      for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds[lds_index<L, C>(c, j + r * (L / R2))];
     }
     run_stage<S + 1, L, C, MODE>(v, lds, a, out, col0, t);
   } else {
@@ -121,7 +154,7 @@ __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const
   constexpr int R0 = radix_at(L, 0);
   constexpr int NB0 = P / R0;
   constexpr bool cf = ModeTraits<MODE>::in_cf;
-  extern __shared__ float2 lds[];
+  extern __shared__ float lds[];
   const unsigned blk = psx::xcd_logical_block(blockIdx.x, gridDim.x);
   const unsigned col0 = blk * C;
   float2* out = a.out + (int64_t)blockIdx.y * a.out_sub + (int64_t)blockIdx.z * a.out_tr;
@@ -229,7 +262,8 @@ struct KernelInfo {
 
 template <int L, int C, int MODE, bool RS>
 constexpr KernelInfo kinfo1() {
-  return KernelInfo{L, C, MODE, RS, &k_fft_pass<L, C, MODE, RS>, C * L / points_per_thread(L), lds_bytes(L, C)};
+  return KernelInfo{L, C, MODE, RS, &k_fft_pass<L, C, MODE, RS>, C * L / points_per_thread(L),
+                    lds_bytes(L, C, split_planes(MODE))};
 }
 // single-pass kernels (C = 1): contiguous, plain and resampling
 #define PSX_K1(L) kinfo1<L, 1, PM_CONTIG, false>(), kinfo1<L, 1, PM_CONTIG, true>()
@@ -379,11 +413,13 @@ bool pick_tiles(Decomp& d) {
   for (int p = 0; p < d.npass; ++p) {
     if (d.C[p] > 0 && find_kernel(d.L[p], d.C[p])) continue;
     const int pref[] = {16, 8, 4, 2};
-    const uint32_t cap = (p == d.npass - 1 && d.npass > 1) ? 160u * 1024u : 80u * 1024u;
+    const bool last = (p == d.npass - 1 && d.npass > 1);
+    const uint32_t cap = last ? 160u * 1024u : 80u * 1024u;
+    const int mode = last ? PM_ROWS_TO_COLS : PM_COLS_TW;
     d.C[p] = 0;
     for (int c : pref) {
-      const KernelInfo* k = find_kernel(d.L[p], c);
-      if (k && k->lds <= cap) {
+      const KernelInfo* k = find_kernel(d.L[p], c, mode, false);
+      if (k && (uint32_t)k->lds <= cap) {
         d.C[p] = c;
         break;
       }
