@@ -84,7 +84,8 @@ def cpu_baseline(O, W, w, series0, accs, threads, target_s=20.0):
                       (t_pro, n, len(accs), t_acc, len(accs))}
 
 
-def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16, dms=(2, 8), log=log):
+def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16, dms=(2, 8), schedules=None,
+                 log=log):
     """Bounded Poisson stream of segment-search requests through the server.
 
     Each request is one beam: DM count uniform in dms (default {2..8}: the
@@ -110,7 +111,7 @@ def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16,
         reqs.append((t, w, W.generate_device(w)))
     P.synchronize()
     ntrials = sum(w.ndm * W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi).size for _, w, _ in reqs)
-    with Server(budget, batch=batch) as srv:
+    with Server(budget, batch=batch, schedules=schedules) as srv:
         ids = []
         t0 = time.perf_counter()
         for ta, w, d in reqs:
@@ -139,7 +140,7 @@ def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16,
     return res
 
 
-def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, log=log):
+def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, schedules=None, log=log):
     """Segmented searches of uneven length: ndm DM trials, an equal mix of fft
     sizes 2^22, 3*2^21 and 2^23 at 76 us (each segment 90% of its fft size,
     so the series are zero padded after baseline removal), red noise and
@@ -158,7 +159,7 @@ def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, log=log):
         reqs.append((w, W.generate_device(w)))
     P.synchronize()
     ntrials = sum(w.ndm * W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi).size for w, _ in reqs)
-    with Server(200e9, batch=batch) as srv:
+    with Server(200e9, batch=batch, schedules=schedules) as srv:
         # warm-up: engines for every size class
         for w, d in reqs:
             srv.wait(srv.submit(w.search_args(), d, w.dm_list()[:1], nsamps=w.n_real))
@@ -339,10 +340,11 @@ def main():
     if not a.no_long:
         result["long_segment"] = long_segment(P, W)
     if not a.no_uneven:
-        result["uneven_batched"] = uneven_batched(P, W, batch=batch)
+        result["uneven_batched"] = uneven_batched(P, W, batch=batch, schedules=tuning.table(dev["name"]))
     if not a.no_serve:
         lo, hi = (int(x) for x in a.serve_dms.split(","))
-        result["serving"] = serve_stream(P, W, nreq=a.serve_requests, batch=batch, dms=(lo, hi))
+        result["serving"] = serve_stream(P, W, nreq=a.serve_requests, batch=batch, dms=(lo, hi),
+                                         schedules=tuning.table(dev["name"]))
     print(json.dumps(result), flush=True)
 
 

@@ -269,11 +269,22 @@ int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, c
 /* Serving: continuous batching of segment-search requests             */
 /* ------------------------------------------------------------------ */
 typedef struct psx_server psx_server;
+/* A tuned schedule for one FFT length (the autotuner's output). */
+typedef struct psx_schedule {
+  uint64_t size;      /* FFT length it applies to */
+  uint32_t batch;     /* trials per fused batch (0 = the server's batch) */
+  int32_t fft_npass;  /* 0 = default decomposition */
+  int32_t fft_len[3];
+  int32_t fft_cols[3];
+} psx_schedule;
+
 typedef struct psx_server_config {
   uint64_t hbm_budget_bytes;  /* resident series + engine buffers never exceed this */
   uint32_t batch;             /* trials per fused batch (default 16) */
   uint32_t max_active;        /* admitted requests at once (0 = budget only) */
   int32_t use_graph;          /* graph-launched batches */
+  uint32_t nschedules;        /* per-length schedules (copied at create; may be 0) */
+  const psx_schedule* schedules;
 } psx_server_config;
 
 /* One beam: ndm dedispersed series of nsamps samples (host or device memory,

@@ -23,13 +23,14 @@ namespace {
 constexpr unsigned kB = 256;
 
 // untangle + form_interpolated + normalise (src/kernels.cu:238-259,496-507).
-// Lane l of a wave owns lower bin k = base + l (k <= M/2) and its mirror
-// M-k.  The pair (Z[k], Z[M-k]) untangles into both X[k] and X[M-k] (the two
-// halves of the plain untangle pass, same arithmetic); loads of Z[k] and
-// Z[M-k] are coalesced (the second in descending order).  Interbinning needs
-// X[k-1] and X[M-k-1] = upper X of pair k+1: both come from the neighbouring
-// lane by a wave shuffle, and the wave's edge lanes untangle that pair
-// themselves.
+// The half-length spectrum pair (Z[k], Z[M-k]), k <= M/2, untangles into both
+// X[k] and X[M-k] (the two halves of the plain untangle pass, same
+// arithmetic).  Interbinning needs X[k-1] beside X[k], and X[M-k-1] (the
+// upper half of pair k+1) beside X[M-k].  A wave holds 64 consecutive pairs
+// k = kw-1 .. kw+62 (one lane per pair, both loads coalesced, the mirror row
+// descending) and takes every neighbour from the adjacent lane by a shuffle:
+// lanes 1..63 write P[k], lanes 0..62 write P[M-k], and consecutive waves
+// advance by 63 pairs, so no lane ever loads a neighbour itself.
 __device__ inline float normalised(float2 x, float2 prev, float mean_n, float std_n) {
   float v = psx::interbin(x, prev);
   v -= mean_n;
@@ -37,22 +38,12 @@ __device__ inline float normalised(float2 x, float2 prev, float mean_n, float st
   return v;
 }
 
-// Lower (X[k]) or upper (X[M-k]) half of untangle pair k, 0 <= k <= M/2.
-__device__ inline float2 pair_half(const float2* __restrict__ z, uint32_t m, uint32_t k, const RootTable& rt,
-                                   bool upper) {
-  const float2 zk = z[k];
-  const float2 zm = z[k == 0 ? 0 : m - k];
-  const float2 w = root(rt, (uint64_t)k);
-  if (!upper || 2 * k == m) return untangle_one(zk, zm, w);
-  return untangle_one(zm, zk, make_float2(-w.x, w.y));
-}
-
 __device__ inline float2 shfl_f2(float2 v, int src) {
   return make_float2(__shfl(v.x, src), __shfl(v.y, src));
 }
 
-// Each thread takes Q pairs per iteration (stride kB apart, so every wave's
-// lanes stay on consecutive bins) and issues all 2Q loads before any math.
+constexpr uint32_t kStep = 63;  // pairs written per wave
+
 template <int Q>
 __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, float* __restrict__ P, uint64_t M,
                                                   int64_t pstride, RootTable rt, const TrialDesc* __restrict__ descs) {
@@ -63,39 +54,41 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
   const float std_n = descs[tr].std_n;
   const uint32_t m = (uint32_t)M, half = m / 2;
   const int lane = (int)(threadIdx.x & 63u);
-  const uint32_t stride = gridDim.x * kB * Q;
-  for (uint32_t base = blockIdx.x * kB * Q; base <= half; base += stride) {
+  const uint32_t waves = gridDim.x * (kB / 64);
+  const uint32_t wave0 = blockIdx.x * (kB / 64) + (threadIdx.x >> 6);
+  const uint32_t nwchunks = (half + 1 + kStep - 1) / kStep;  // waves of 63 lower outputs cover 0..half
+  for (uint32_t c = wave0; c < nwchunks; c += waves * Q) {
     float2 zk[Q], zm[Q];
+    int32_t kq[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const uint32_t k = base + (uint32_t)q * kB + threadIdx.x;
+      const uint32_t cw = c + (uint32_t)q * waves;
+      const int32_t k = (int32_t)(cw * kStep) + lane - 1;
+      kq[q] = k;
       zk[q] = zm[q] = make_float2(0.f, 0.f);
-      if (k <= half) {
+      if (cw < nwchunks && k >= 0 && (uint32_t)k <= half) {
         zk[q] = z[k];
-        zm[q] = z[k == 0 ? 0 : m - k];
+        zm[q] = z[k == 0 ? 0 : m - (uint32_t)k];
       }
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const uint32_t k = base + (uint32_t)q * kB + threadIdx.x;
-      const bool valid = k <= half;
+      const int32_t k = kq[q];
+      const bool valid = (c + (uint32_t)q * waves) < nwchunks && k >= 0 && (uint32_t)k <= half;
       float2 xl = make_float2(0.f, 0.f), xu = make_float2(0.f, 0.f);
       if (valid) {
         const float2 w = root(rt, (uint64_t)k);
         xl = untangle_one(zk[q], zm[q], w);
         // W_n^(M-k) = -conj(W_n^k); the midpoint is its own mirror and keeps
         // the lower formula, as the plain pass writes it
-        xu = (k == half) ? xl : untangle_one(zm[q], zk[q], make_float2(-w.x, w.y));
+        xu = ((uint32_t)k == half) ? xl : untangle_one(zm[q], zk[q], make_float2(-w.x, w.y));
       }
-      float2 xl_prev = shfl_f2(xl, lane > 0 ? lane - 1 : 0);
-      float2 xu_next = shfl_f2(xu, lane < 63 ? lane + 1 : 63);
+      // X[k-1] from the lane below (zero below bin 0), X[M-k-1] from above
+      const float2 xl_prev = shfl_f2(xl, lane > 0 ? lane - 1 : 0);
+      const float2 xu_next = shfl_f2(xu, lane < 63 ? lane + 1 : 63);
       if (valid) {
-        if (lane == 0) xl_prev = k == 0 ? make_float2(0.f, 0.f) : pair_half(z, m, k - 1, rt, false);
-        if (k < half) {
-          if (lane == 63) xu_next = pair_half(z, m, k + 1, rt, true);
-          p[m - k] = normalised(xu, xu_next, mean_n, std_n);
-        }
-        p[k] = normalised(xl, xl_prev, mean_n, std_n);
+        if (lane > 0) p[k] = normalised(xl, xl_prev, mean_n, std_n);
+        if (lane < 63 && (uint32_t)k < half) p[m - (uint32_t)k] = normalised(xu, xu_next, mean_n, std_n);
       }
     }
   }
@@ -166,7 +159,8 @@ int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, con
                      const TrialDesc* d_descs, uint32_t ntr, hipStream_t s) {
   if (M < 8 || M > 0x7FFFFFFFull) return psx::fail(PSX_ERR_ARG, "spec_post: unsupported length");
   constexpr int Q = 4;
-  const unsigned gx = psx::grid_for(M / 2 + 1, kB * Q, 2048);
+  const uint64_t chunks = (M / 2 + 1 + kStep - 1) / kStep;  // one wave each
+  const unsigned gx = psx::grid_for(chunks * 64, kB * Q, 2048);
   k_spec_post<Q><<<dim3(gx, ntr), kB, 0, s>>>(Z, P, M, pstride, rt, d_descs);
   PSX_CHECK_LAUNCH("k_spec_post");
   return PSX_OK;

@@ -41,10 +41,12 @@ __host__ __device__ constexpr int points_per_thread(int L) {
 __host__ __device__ constexpr int ns_before(int L, int s) { return s == 0 ? 1 : ns_before(L, s - 1) * radix_at(L, s - 1); }
 // LDS row pitch in float2: one pad slot per 16 elements plus one per row.
 __host__ __device__ constexpr int lds_pitch(int L, int C) { return L + L / 16 + (C > 1 ? 1 : 0); }
-// Stage exchanges move one float plane at a time (real parts, then imaginary
-// parts): one float of LDS per point, half of a complex exchange, so twice
-// the workgroups fit on a CU.
-__host__ __device__ constexpr bool split_planes(int /*mode*/) { return true; }
+// Passes over strided columns exchange one float plane at a time between
+// stages (real parts, then imaginary parts): one float of LDS per point,
+// half of a complex exchange, so twice the workgroups fit on a CU.  Row
+// passes exchange whole complex values (measured faster there: their extra
+// barriers cost more than the occupancy gains).
+__host__ __device__ constexpr bool split_planes(int mode) { return mode == 1; }
 __host__ __device__ constexpr int lds_bytes(int L, int C, bool split) {
   return C * lds_pitch(L, C) * (split ? 4 : 8);
 }

@@ -108,6 +108,8 @@ double now_s() {
 
 struct psx_server {
   psx_server_config cfg;
+  std::map<uint64_t, psx_schedule> schedules;  // tuned schedule per FFT length
+  std::map<uint64_t, psx::fft::Decomp> decomps;
   std::mutex mu;
   std::condition_variable cv;
   std::thread sched;
@@ -123,6 +125,10 @@ struct psx_server {
 
   void loop();
   bool admit_one(Request* r);
+  uint32_t batch_for(uint64_t size) const {
+    auto it = schedules.find(size);
+    return (it != schedules.end() && it->second.batch) ? it->second.batch : cfg.batch;
+  }
   void finish(Request* r);
   void fail_request(Request* r, int code, const std::string& msg);
   EngineState* engine_for(const psx_search_args& a, bool create, uint64_t* need);
@@ -147,15 +153,17 @@ EngineState* psx_server::engine_for(const psx_search_args& a, bool create, uint6
     if (need) *need = 0;
     return it->second.get();
   }
-  const uint64_t eb = Engine::device_bytes(a.size, cfg.batch, 1u << 20);
+  const uint64_t eb = Engine::device_bytes(a.size, batch_for(a.size), 1u << 20);
   if (need) *need = eb;
   if (!create) return nullptr;
   std::unique_ptr<EngineState> es(new EngineState);
   if (hipStreamCreateWithFlags(&es->stream, hipStreamNonBlocking) != hipSuccess)
     throw std::runtime_error("stream creation failed");
   EngineOptions eo;
-  eo.batch = cfg.batch;
+  eo.batch = batch_for(a.size);
   eo.use_graph = cfg.use_graph != 0;
+  auto dc = decomps.find(a.size);
+  if (dc != decomps.end()) eo.decomp = &dc->second;
   es->eng.reset(new Engine(a, eo, es->stream));
   es->bytes = eb;
   es->last_used = clk::now();
@@ -357,9 +365,29 @@ void psx_server::loop() {
 extern "C" int psx_server_create(const psx_server_config* cfg, psx_server** out) {
   PSX_REQUIRE(cfg && out, "psx_server_create: null argument");
   PSX_REQUIRE(cfg->hbm_budget_bytes > 0, "psx_server_create: zero budget");
+  PSX_REQUIRE(!cfg->nschedules || cfg->schedules, "psx_server_create: null schedules");
   psx_server* s = new psx_server;
   s->cfg = *cfg;
   if (!s->cfg.batch) s->cfg.batch = 16;
+  for (uint32_t i = 0; i < cfg->nschedules; ++i) {
+    const psx_schedule& sc = cfg->schedules[i];
+    if (sc.batch > 1024 || sc.fft_npass < 0 || sc.fft_npass > 3) {
+      delete s;
+      return psx::fail(PSX_ERR_ARG, "psx_server_create: bad schedule for size %llu", (unsigned long long)sc.size);
+    }
+    s->schedules[sc.size] = sc;
+    if (sc.fft_npass > 0) {
+      psx::fft::Decomp d;
+      d.npass = sc.fft_npass;
+      for (int q = 0; q < 3; ++q) {
+        d.L[q] = sc.fft_len[q];
+        d.C[q] = sc.fft_cols[q];
+      }
+      s->decomps[sc.size] = d;
+    }
+  }
+  s->cfg.schedules = nullptr;  // copied above; the caller's array is not kept
+  s->cfg.nschedules = 0;
   (void)hipGetDevice(&s->device);
   s->sched = std::thread([s]() { s->loop(); });
   *out = s;
@@ -371,7 +399,7 @@ extern "C" int psx_server_submit(psx_server* s, const psx_request* req, uint64_t
   int rc = psx::search::validate_args(&req->args);
   if (rc) return rc;
   PSX_REQUIRE(req->series && req->dm_list && req->ndm > 0 && req->nsamps > 0, "psx_server_submit: bad series");
-  const uint64_t need = request_bytes(*req) + Engine::device_bytes(req->args.size, s->cfg.batch, 1u << 20);
+  const uint64_t need = request_bytes(*req) + Engine::device_bytes(req->args.size, s->batch_for(req->args.size), 1u << 20);
   if (need > s->cfg.hbm_budget_bytes)
     return psx::fail(PSX_ERR_BUDGET, "request needs %llu bytes, budget is %llu", (unsigned long long)need,
                      (unsigned long long)s->cfg.hbm_budget_bytes);

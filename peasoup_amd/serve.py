@@ -12,9 +12,14 @@ import peasoup_amd as P
 from peasoup_amd.workloads import CAND_DTYPE, SearchArgs, top_level
 
 
+class Schedule(C.Structure):
+    _fields_ = [("size", C.c_uint64), ("batch", C.c_uint32), ("fft_npass", C.c_int32), ("fft_len", C.c_int32 * 3),
+                ("fft_cols", C.c_int32 * 3)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("batch", C.c_uint32), ("max_active", C.c_uint32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("nschedules", C.c_uint32), ("schedules", C.POINTER(Schedule))]
 
 
 class Request(C.Structure):
@@ -53,9 +58,26 @@ def _status(st):
             "latency": st.t_done - st.t_submit if st.state >= 2 else None}
 
 
+def schedules_from(table):
+    """psx_schedule array from {size: {"batch": B, "decomp": [(L, C), ...] or None}}."""
+    arr = (Schedule * max(len(table), 1))()
+    for i, (n, e) in enumerate(sorted(table.items())):
+        arr[i].size = int(n)
+        arr[i].batch = int(e.get("batch") or 0)
+        dec = e.get("decomp") or []
+        arr[i].fft_npass = len(dec)
+        for q, (L, Cc) in enumerate(dec):
+            arr[i].fft_len[q] = int(L)
+            arr[i].fft_cols[q] = int(Cc)
+    return arr, len(table)
+
+
 class Server:
-    def __init__(self, budget_bytes, batch=16, max_active=0, use_graph=True):
-        self.cfg = ServerConfig(int(budget_bytes), batch, max_active, 1 if use_graph else 0)
+    def __init__(self, budget_bytes, batch=16, max_active=0, use_graph=True, schedules=None):
+        """schedules: optional {fft size: {"batch", "decomp"}} (the tuned table)."""
+        arr, n = schedules_from(schedules or {})
+        self.cfg = ServerConfig(int(budget_bytes), batch, max_active, 1 if use_graph else 0, n,
+                                C.cast(arr, C.POINTER(Schedule)))
         h = C.c_void_p()
         P.check(P.lib().psx_server_create(C.byref(self.cfg), C.byref(h)))
         self.h = h.value

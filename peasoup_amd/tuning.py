@@ -38,6 +38,15 @@ def lookup(n, device):
     return {}
 
 
+def table(device):
+    """Every tuned size for the device: {n: {"batch", "decomp"}} (user cache first)."""
+    out = {}
+    for t in (_load(SHIPPED), _load(CACHE)):
+        for n, e in t.get(device, {}).items():
+            out[int(n)] = {"batch": e["batch"], "decomp": [tuple(x) for x in e["decomp"]] if e.get("decomp") else None}
+    return out
+
+
 def store(n, device, entry, path=CACHE):
     table = _load(path)
     table.setdefault(device, {})[str(n)] = entry

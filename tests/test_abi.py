@@ -42,3 +42,16 @@ def test_fft_plan_rejects_unsupported_sizes_before_device_work():
     assert L.psx_fft_plan_create(1000, 0, 1, C.byref(p)) == 1
     assert L.psx_fft_plan_create(1 << 29, 0, 1, C.byref(p)) == 1
     assert L.psx_fft_plan_create(4096, 7, 1, C.byref(p)) == 1
+
+
+def test_server_rejects_bad_schedule_without_device():
+    import ctypes as C
+    import peasoup_amd as P
+    from peasoup_amd.serve import Schedule, ServerConfig
+    arr = (Schedule * 1)()
+    arr[0].size = 1 << 20
+    arr[0].fft_npass = 7
+    cfg = ServerConfig(1 << 30, 16, 0, 1, 1, C.cast(arr, C.POINTER(Schedule)))
+    h = C.c_void_p()
+    assert P.lib().psx_server_create(C.byref(cfg), C.byref(h)) == 1
+    assert b"bad schedule" in P.lib().psx_last_error_string()

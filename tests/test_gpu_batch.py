@@ -106,6 +106,20 @@ def test_zapped_search(psx, orc):
     assert near50(unzapped) and not near50(got)
 
 
+@pytest.mark.parametrize("nharm", [1, 2, 3, 5])
+def test_batched_equals_worker_every_fold_count(psx, nharm):
+    """Each fold count has its own fused harmonic kernel."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("h", 2, 1 << 16, 64e-6, 0x5EA5A0 + nharm, 0.0, -30.0, 30.0, nharmonics=nharm, min_snr=6.0,
+                   injections=[W.Injection(1, 3.3e-3, 5.0, 15.0, 0.1)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs, forest=True)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(4), forest=True)
+    assert same(got, ref) and len(got) == len(ref)
+    assert len(got) > 0 and set(got["nh"]) <= set(range(nharm + 1))
+
+
 def test_batched_device_input_and_plan(psx, cfg1):
     import peasoup_amd.workloads as W
     w, series, accs = cfg1

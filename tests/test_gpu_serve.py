@@ -111,3 +111,20 @@ def test_request_with_zap_list(psx, reqs):
         got1, got2 = srv.result(i1), srv.result(i2)
     assert same(got1, direct) and same(got2, plain)
     assert not same(direct, plain)
+
+
+def test_tuned_schedule_gives_same_results(psx, reqs):
+    """A per-length schedule (batch and FFT decomposition) changes only how
+    the trials are launched, never the candidates."""
+    from peasoup_amd.serve import Server
+    w, series = reqs[0]  # 2^17 samples
+    with Server(64 << 30, batch=8) as srv:
+        rid = srv.submit(w.search_args(), series, w.dm_list())
+        assert srv.wait(rid)["state"] == "done"
+        ref = srv.result(rid, forest=True)
+    sched = {w.size: {"batch": 3, "decomp": [(64, 16), (1024, 8)]}}
+    with Server(64 << 30, batch=8, schedules=sched) as srv:
+        rid = srv.submit(w.search_args(), series, w.dm_list())
+        assert srv.wait(rid)["state"] == "done"
+        got = srv.result(rid, forest=True)
+    assert same(got, ref) and len(got) == len(ref)
