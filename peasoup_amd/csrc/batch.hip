@@ -98,41 +98,101 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
 // (src/kernels.cu:391-423) for the fundamental and every fold, each over
 // its own PeakFinder window (peakfinder.hpp:78-83).  Survivors are appended
 // to `out` with one atomic per wave; the host orders them by bin.
+//
+// A wave covers kHarmWaveBins consecutive bins; lane l holds the D bins
+// base + l + 64d.  Fold k's harmonic j of bin i is bin (j*i + 2^(k-1)) >> k,
+// and with i = i0 + 64d that is ((j*i0 + 2^(k-1)) >> k) + d*64j/2^k exactly
+// (2^k divides 64), so each harmonic costs one index per lane and D loads at
+// constant instruction offsets from it: the gathers stay coalesced across
+// lanes and the index arithmetic of harmonic_sums32 drops out of the loop.
+// The running sums take the same addends in the same order as
+// harmonic_sums32, so the thresholded values are bit-identical to it.  The
+// loop only decides whether any value of a lane crosses the threshold; a
+// wave with a crossing recomputes its bins one by one with harmonic_sums32,
+// applies the fold windows and appends.
+template <int K, int J, int D>
+__device__ inline void add_harmonic(const PSX_GLOBAL float* p, uint32_t i0, float (&val)[D]) {
+  constexpr uint32_t step = (64u * J) >> K;
+  const PSX_GLOBAL float* q = p + ((J * i0 + (1u << (K - 1))) >> K);
+#pragma unroll
+  for (int d = 0; d < D; ++d) val[d] += q[d * step];
+}
+// odd harmonics J, J+2, ... below JEND of fold K, in increasing order
+template <int K, int J, int JEND, int D>
+__device__ inline void add_harmonics(const PSX_GLOBAL float* p, uint32_t i0, float (&val)[D]) {
+  add_harmonic<K, J, D>(p, i0, val);
+  if constexpr (J + 2 < JEND) add_harmonics<K, J + 2, JEND, D>(p, i0, val);
+}
+template <int K, int D>
+__device__ inline bool fold_crosses(const float (&val)[D], float thr) {
+  bool any = false;
+#pragma unroll
+  for (int d = 0; d < D; ++d) any |= fold_scale<K>(val[d]) > thr;
+  return any;
+}
+
 template <int NF>
 __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, int64_t pstride, Windows w,
                                                    const TrialDesc* __restrict__ descs, Cand16* __restrict__ out,
                                                    uint32_t cap, uint32_t* __restrict__ count) {
-  constexpr int Q = 4;  // bins per thread per iteration (independent gather chains)
+  constexpr int D = kHarmWaveBins / 64;
   const uint32_t tr = blockIdx.y;
   const float* p = P + (int64_t)tr * pstride;
+  const PSX_GLOBAL float* pg = (const PSX_GLOBAL float*)p;
   const unsigned lane = threadIdx.x & 63u;
-  const int32_t step = (int32_t)(gridDim.x * kB * Q);
-  const int32_t hi = (int32_t)w.hi;
-  for (int32_t base = (int32_t)w.lo + (int32_t)(blockIdx.x * kB * Q); base < hi; base += step) {
-    float s[Q][6];
+  const uint32_t hi = (uint32_t)w.hi;
+  const uint32_t wstep = gridDim.x * (kB / 64) * kHarmWaveBins;
+  for (uint32_t base = (uint32_t)w.lo + (blockIdx.x * (kB / 64) + (threadIdx.x >> 6)) * kHarmWaveBins; base < hi;
+       base += wstep) {
+    const uint32_t i0 = base + lane;
+    float val[D];
     bool any = false;
+    {
+      const PSX_GLOBAL float* q = pg + i0;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int32_t i = base + (int32_t)threadIdx.x + q * (int32_t)kB;
-      if (i < hi) {
-        float f[5];
-        psx::harmonic_sums32<NF>(p, (uint32_t)i, f);
-        s[q][0] = p[i];
+      for (int d = 0; d < D; ++d) val[d] = q[64 * d];
 #pragma unroll
-        for (int k = 0; k < NF; ++k) s[q][k + 1] = f[k];
-#pragma unroll
-        for (int nh = 0; nh <= NF; ++nh) any |= (i >= w.start[nh]) & (i < w.end[nh]) & (s[q][nh] > w.thr);
-      }
+      for (int d = 0; d < D; ++d) any |= val[d] > w.thr;
+    }
+    if constexpr (NF > 0) {
+      add_harmonic<1, 1, D>(pg, i0, val);
+      any |= fold_crosses<1, D>(val, w.thr);
+    }
+    if constexpr (NF > 1) {
+      add_harmonic<2, 3, D>(pg, i0, val);  // 3/4 before 1/4, as the reference
+      add_harmonic<2, 1, D>(pg, i0, val);
+      any |= fold_crosses<2, D>(val, w.thr);
+    }
+    if constexpr (NF > 2) {
+      add_harmonics<3, 1, 8, D>(pg, i0, val);
+      any |= fold_crosses<3, D>(val, w.thr);
+    }
+    if constexpr (NF > 3) {
+      add_harmonics<4, 1, 16, D>(pg, i0, val);
+      any |= fold_crosses<4, D>(val, w.thr);
+    }
+    if constexpr (NF > 4) {
+      add_harmonics<5, 1, 32, D>(pg, i0, val);
+      any |= fold_crosses<5, D>(val, w.thr);
     }
     // survivors are rare: one ballot decides whether this wave appends at all
     if (__ballot(any) == 0) continue;
     const uint32_t slot = descs[tr].slot;
+#pragma unroll 1
+    for (int d = 0; d < D; ++d) {
+      const uint32_t i = i0 + 64u * (uint32_t)d;
+      const bool valid = i < hi;
+      float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (valid) {
+        float f[5];
+        psx::harmonic_sums32<NF>(p, i, f);
+        s[0] = p[i];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int32_t i = base + (int32_t)threadIdx.x + q * (int32_t)kB;
+        for (int k = 0; k < NF; ++k) s[k + 1] = f[k];
+      }
 #pragma unroll
       for (int nh = 0; nh <= NF; ++nh) {
-        const bool hit = (i < hi) && i >= w.start[nh] && i < w.end[nh] && s[q][nh] > w.thr;
+        const bool hit = valid && (int32_t)i >= w.start[nh] && (int32_t)i < w.end[nh] && s[nh] > w.thr;
         const uint64_t mask = __ballot(hit);
         if (mask) {
           const int leader = __ffsll((long long)mask) - 1;
@@ -141,7 +201,7 @@ __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, 
           b = __shfl(b, leader);
           if (hit) {
             const uint32_t pos = b + psx::mbcnt64(mask);
-            if (pos < cap) out[pos] = Cand16{slot, (uint32_t)nh, i, s[q][nh]};
+            if (pos < cap) out[pos] = Cand16{slot, (uint32_t)nh, (int32_t)i, s[nh]};
           }
         }
       }
@@ -171,7 +231,10 @@ int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_
   if (w.hi <= w.lo) return PSX_OK;
   // 32-bit bin arithmetic: j*i < 32 * 2^27
   if (w.hi > (1ll << 27)) return psx::fail(PSX_ERR_UNSUPPORTED, "harm_peaks: more than 2^27 bins");
-  const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), 4 * kB, 2048);
+  // a wave reads up to kHarmWaveBins - 1 bins past w.hi: the spectrum
+  // buffer holds that many floats beyond the last trial's row
+  if (w.lo < 0 || w.hi > pstride) return psx::fail(PSX_ERR_ARG, "harm_peaks: window outside the spectrum row");
+  const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), (uint64_t)kHarmWaveBins * (kB / 64), 2048);
   const dim3 g(gx, ntr);
   switch (nfolds) {
     case 1: k_harm_peaks<1><<<g, kB, 0, s>>>(P, pstride, w, d_descs, out, cap, count); break;

@@ -42,8 +42,8 @@ struct ModeTraits {
 };
 
 template <int S, int L, int C, int MODE>
-__device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, const PassArgs& a, float2* out,
-                                 unsigned col0, int t) {
+__device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2 (&tw)[points_per_thread(L)], float* lds,
+                                 const PassArgs& a, float2* out, unsigned col0, int t) {
   constexpr int P = points_per_thread(L);
   constexpr int NT = C * L / P;
   constexpr int NSTG = num_stages(L);
@@ -52,6 +52,7 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
   constexpr int Ns = ns_before(L, S);
   constexpr bool LAST = (S == NSTG - 1);
   constexpr bool cf = (S == 0) ? ModeTraits<MODE>::in_cf : (LAST ? ModeTraits<MODE>::out_cf : false);
+  constexpr int TWS = twiddle_source(L, C, MODE);
 
   // twiddle + butterfly
 #pragma unroll
@@ -62,7 +63,16 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
       const int k = j % Ns;
       constexpr int TS = L / (Ns * R);
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], a.twL[k * r * TS]);
+      for (int r = 1; r < R; ++r) {
+        float2 w;
+        if constexpr (TWS == TW_PREFETCH)
+          w = tw[b * (R - 1) + r - 1];
+        else if constexpr (TWS == TW_LDS)
+          w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[k * r * TS];
+        else
+          w = ldg2(a.twL, (uint32_t)(k * r * TS));
+        v[b * R + r] = cmul(v[b * R + r], w);
+      }
     }
     dft<R>(&v[b * R]);
   }
@@ -71,6 +81,21 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
     constexpr int R2 = radix_at(L, S + 1);
     constexpr int NB2 = P / R2;
     constexpr bool cf2 = (S + 1 == NSTG - 1) ? ModeTraits<MODE>::out_cf : false;
+    if constexpr (TWS == TW_PREFETCH) {
+      // next stage's twiddles, in flight during the exchange below
+      constexpr int Ns2 = ns_before(L, S + 1);
+      constexpr int TS2 = L / (Ns2 * R2);
+#pragma unroll
+      for (int b = 0; b < NB2; ++b) {
+        int c, j;
+        bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
+        const int k = j % Ns2;
+#pragma unroll
+        for (int r = 1; r < R2; ++r) tw[b * (R2 - 1) + r - 1] = ldg2(a.twL, (uint32_t)(k * r * TS2));
+      }
+    }
+    constexpr int WS = lds_step(Ns, R);            // write run: idxD + r*Ns
+    constexpr int RSTEP = lds_step(L / R2, R2);    // read run: j + r*(L/R2)
     if constexpr (!split_planes(MODE)) {
       float2* lds2 = reinterpret_cast<float2*>(lds);
       if (S > 0) __syncthreads();
@@ -80,16 +105,18 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
         bf_coords<L, C, R, NT>(b, t, cf, c, j);
         const int k = j % Ns;
         const int idxD = (j / Ns) * Ns * R + k;
+        const int w0 = lds_index<L, C>(c, idxD);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds2[lds_index<L, C>(c, idxD + r * Ns)] = v[b * R + r];
+        for (int r = 0; r < R; ++r) lds2[WS ? w0 + r * WS : lds_index<L, C>(c, idxD + r * Ns)] = v[b * R + r];
       }
       __syncthreads();
 #pragma unroll
       for (int b = 0; b < NB2; ++b) {
         int c, j;
         bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
+        const int r0 = lds_index<L, C>(c, j);
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds2[lds_index<L, C>(c, j + r * (L / R2))];
+        for (int r = 0; r < R2; ++r) v[b * R2 + r] = lds2[RSTEP ? r0 + r * RSTEP : lds_index<L, C>(c, j + r * (L / R2))];
       }
     } else {
     // real plane, then imaginary plane: each point's component is written
@@ -103,17 +130,20 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
         bf_coords<L, C, R, NT>(b, t, cf, c, j);
         const int k = j % Ns;
         const int idxD = (j / Ns) * Ns * R + k;
+        const int w0 = lds_index<L, C>(c, idxD);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_index<L, C>(c, idxD + r * Ns)] = plane ? v[b * R + r].y : v[b * R + r].x;
+        for (int r = 0; r < R; ++r)
+          lds[WS ? w0 + r * WS : lds_index<L, C>(c, idxD + r * Ns)] = plane ? v[b * R + r].y : v[b * R + r].x;
       }
       __syncthreads();
 #pragma unroll
       for (int b = 0; b < NB2; ++b) {
         int c, j;
         bf_coords<L, C, R2, NT>(b, t, cf2, c, j);
+        const int r0 = lds_index<L, C>(c, j);
 #pragma unroll
         for (int r = 0; r < R2; ++r) {
-          const float x = lds[lds_index<L, C>(c, j + r * (L / R2))];
+          const float x = lds[RSTEP ? r0 + r * RSTEP : lds_index<L, C>(c, j + r * (L / R2))];
           if (plane)
             v[b * R2 + r].y = x;
           else
@@ -122,7 +152,7 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
       }
     }
     }
-    run_stage<S + 1, L, C, MODE>(v, lds, a, out, col0, t);
+    run_stage<S + 1, L, C, MODE>(v, tw, lds, a, out, col0, t);
   } else {
     // element offsets inside one transform are < n/2 <= 2^27: 32-bit math
     const uint32_t ocol = (uint32_t)a.out_col, oel = (uint32_t)a.out_elem;
@@ -138,17 +168,18 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float* lds, 
         float2 val = v[b * R + r];
         if constexpr (ModeTraits<MODE>::tw4) {
           const uint32_t x = cg * e;
-          val = cmul(val, cmul(a.tw4_hi[x >> a.tw4_shift], a.tw4_lo[x & a.tw4_mask]));
+          val = cmul(val, cmul(ldg2(a.tw4_hi, x >> a.tw4_shift), ldg2(a.tw4_lo, x & a.tw4_mask)));
         }
         if (a.out_conj) val.y = -val.y;
-        out[o0 + (uint32_t)(r * (L / R)) * oel] = val;
+        stg2(out, o0 + (uint32_t)(r * (L / R)) * oel, val);
       }
     }
   }
 }
 
 template <int L, int C, int MODE, bool RS>
-__global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const PassArgs a) {
+__global__ void __launch_bounds__(C * L / points_per_thread(L))
+__attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(const PassArgs a) {
   constexpr int P = points_per_thread(L);
   constexpr int NT = C * L / P;
   constexpr int R0 = radix_at(L, 0);
@@ -161,6 +192,12 @@ __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const
   const int t = threadIdx.x;
   const uint32_t icol = (uint32_t)a.in_col, iel = (uint32_t)a.in_elem;
   float2 v[P];
+  float2 tw[P];
+  if constexpr (twiddle_source(L, C, MODE) == TW_LDS) {
+    // the first use follows the stage-0 exchange and its barriers
+    float2* ltw = reinterpret_cast<float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4);
+    for (int i = t; i < L; i += NT) ltw[i] = ldg2(a.twL, (uint32_t)i);
+  }
   if constexpr (RS) {
     // resampleII fused into the first read (src/kernels.cu:342-353); the
     // complex input m is the sample pair (2m, 2m+1) of the resampled series
@@ -178,8 +215,8 @@ __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
         const uint32_t m = m0 + (uint32_t)(r * (L / R0)) * iel;
-        v[b * R0 + r] = make_float2(tim[psx::resample_index32(2 * m, n, dn, af)],
-                                    tim[psx::resample_index32(2 * m + 1, n, dn, af)]);
+        v[b * R0 + r] = make_float2(ldg1(tim, psx::resample_index32(2 * m, n, dn, af)),
+                                    ldg1(tim, psx::resample_index32(2 * m + 1, n, dn, af)));
       }
     }
   } else {
@@ -191,13 +228,13 @@ __global__ void __launch_bounds__(C * L / points_per_thread(L)) k_fft_pass(const
       const uint32_t o0 = (col0 + (uint32_t)c) * icol + (uint32_t)j * iel;
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
-        float2 x = in[o0 + (uint32_t)(r * (L / R0)) * iel];
+        float2 x = ldg2(in, o0 + (uint32_t)(r * (L / R0)) * iel);
         if (a.in_conj) x.y = -x.y;
         v[b * R0 + r] = x;
       }
     }
   }
-  run_stage<0, L, C, MODE>(v, lds, a, out, col0, t);
+  run_stage<0, L, C, MODE>(v, tw, lds, a, out, col0, t);
 }
 
 // --------------------------------------------------------- R2C untangle
@@ -263,7 +300,7 @@ struct KernelInfo {
 template <int L, int C, int MODE, bool RS>
 constexpr KernelInfo kinfo1() {
   return KernelInfo{L, C, MODE, RS, &k_fft_pass<L, C, MODE, RS>, C * L / points_per_thread(L),
-                    lds_bytes(L, C, split_planes(MODE))};
+                    pass_lds_bytes(L, C, MODE)};
 }
 // single-pass kernels (C = 1): contiguous, plain and resampling
 #define PSX_K1(L) kinfo1<L, 1, PM_CONTIG, false>(), kinfo1<L, 1, PM_CONTIG, true>()

@@ -50,6 +50,70 @@ __host__ __device__ constexpr bool split_planes(int mode) { return mode == 1; }
 __host__ __device__ constexpr int lds_bytes(int L, int C, bool split) {
   return C * lds_pitch(L, C) * (split ? 4 : 8);
 }
+// Waves per SIMD that the LDS footprint allows for a pass (160 KB of LDS and
+// at most 32 waves per CU, 4 SIMDs): the register budget the kernel is
+// compiled for (512 / waves VGPRs), so that registers never cap the
+// workgroups per CU below what the LDS admits.  One resident workgroup per
+// CU idles the CU at every barrier and every load phase; two overlap them.
+// Plans with a radix-3 stage hold 48 points per thread (96 VGPRs of data)
+// and keep the compiler's own register allocation (target 1).
+__host__ __device__ constexpr int wgs_per_cu(int L, int C, int bytes) {
+  const int nt = C * L / points_per_thread(L);
+  const int by_lds = 163840 / bytes;
+  return by_lds * nt > 2048 ? 2048 / nt : by_lds;
+}
+__host__ __device__ constexpr int waves_per_simd(int L, int C, int mode) {
+  const int nt = C * L / points_per_thread(L);
+  const int w = (wgs_per_cu(L, C, lds_bytes(L, C, split_planes(mode))) * ((nt + 63) / 64) + 3) / 4;
+  return points_per_thread(L) > 16 ? 1 : (w > 8 ? 8 : (w < 1 ? 1 : w));
+}
+// Where a pass reads its stage twiddles W_L^e from.  Loading them from the
+// global table at the butterfly that uses them stalls on the load latency
+// after every exchange, so:
+//   TW_LDS       the L-entry table is copied into LDS behind the exchange
+//                buffer when that leaves the workgroups per CU unchanged;
+//   TW_PREFETCH  otherwise, when the register budget is at least 128 VGPRs
+//                and a thread holds at most 16 points, each stage's twiddles
+//                are loaded before the preceding LDS exchange (their latency
+//                hides behind it);
+//   TW_GLOBAL    loads at the point of use.
+enum TwiddleSource { TW_GLOBAL = 0, TW_LDS = 1, TW_PREFETCH = 2 };
+__host__ __device__ constexpr int twiddle_source(int L, int C, int mode) {
+  const int base = lds_bytes(L, C, split_planes(mode));
+  if (num_stages(L) < 2) return TW_GLOBAL;
+  if (base + 8 * L <= 163840 && wgs_per_cu(L, C, base + 8 * L) == wgs_per_cu(L, C, base)) return TW_LDS;
+  if (points_per_thread(L) <= 16 && 512 / waves_per_simd(L, C, mode) >= 128) return TW_PREFETCH;
+  return TW_GLOBAL;
+}
+// Dynamic LDS of a pass kernel: the exchange buffer plus the twiddle table.
+__host__ __device__ constexpr int pass_lds_bytes(int L, int C, int mode) {
+  return lds_bytes(L, C, split_planes(mode)) + (twiddle_source(L, C, mode) == TW_LDS ? 8 * L : 0);
+}
+
+// ------------------------------------------------- global memory access
+// Loads and stores at a workgroup-uniform base plus a 32-bit element index.
+// The base is cast to the global address space and the byte offset is a
+// zero-extended 32-bit value, which selects the SGPR-base + 32-bit
+// VGPR-offset form of global_load/global_store: one VGPR per address and no
+// 64-bit address arithmetic (generic pointers otherwise become flat
+// accesses with 64-bit VGPR addresses).  Offsets stay below 2^32 bytes: an
+// index is below n <= 2^28 samples.
+#define PSX_GLOBAL __attribute__((address_space(1)))
+typedef float psx_f2v __attribute__((ext_vector_type(2)));
+
+__device__ inline float2 ldg2(const float2* base, uint32_t i) {
+  const psx_f2v v = *(const PSX_GLOBAL psx_f2v*)((const PSX_GLOBAL char*)base + (uint64_t)(i * 8u));
+  return make_float2(v.x, v.y);
+}
+__device__ inline float ldg1(const float* base, uint32_t i) {
+  return *(const PSX_GLOBAL float*)((const PSX_GLOBAL char*)base + (uint64_t)(i * 4u));
+}
+__device__ inline void stg2(float2* base, uint32_t i, float2 v) {
+  psx_f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  *(PSX_GLOBAL psx_f2v*)((PSX_GLOBAL char*)base + (uint64_t)(i * 8u)) = w;
+}
 
 // ---------------------------------------------------------- complex math
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -210,6 +274,15 @@ __device__ inline float2 tw4_lookup(const PassArgs& a, uint64_t x) {
 template <int L, int C>
 __device__ inline int lds_index(int c, int e) {
   return c * lds_pitch(L, C) + e + (e >> 4);
+}
+// Index step between the elements e0 + r*K (r < N) of one butterfly's run,
+// or 0 when it is not constant.  Runs are aligned to K*N (Stockham
+// indexing), so a run moves by whole padding groups when K is a multiple of
+// 16 (step K + K/16) and stays inside one 16-element group when K*N divides
+// 16 (step K).  A constant step becomes an instruction offset on one base
+// address instead of a separate address per element.
+__host__ __device__ constexpr int lds_step(int K, int N) {
+  return (K % 16 == 0) ? K + K / 16 : ((K * N <= 16 && 16 % (K * N) == 0) ? K : 0);
 }
 
 template <int L, int C, int R, int NT>

@@ -56,7 +56,7 @@ Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t s
   psx_ok(psx::fft::plan_create_with(N, PSX_FFT_R2C, B, opt.decomp, &plan), "batched R2C plan");
   rt = psx::fft::root_table(plan);
   dalloc(&Z, M * B);
-  dalloc(&P, (uint64_t)pstride * B);
+  dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
   for (Slot& sl : slots) {
     dalloc(&sl.d_desc, B);
     halloc(&sl.h_desc, B);

@@ -96,6 +96,17 @@ __device__ inline float stretch_at(const float* __restrict__ in, uint64_t in_cou
   return (fr > 1e-5f) ? fmaf(fr, b - a, a) : a + 0.f;
 }
 
+// Scale of fold K's running sum (1-based): times 2^-K/2, computed in double
+// for odd K as the reference does (src/kernels.cu:38-98).
+template <int K>
+__device__ inline float fold_scale(float val) {
+  if constexpr (K == 1) return (float)((double)val * 0.70710678118654752440);
+  else if constexpr (K == 2) return val * 0.5f;
+  else if constexpr (K == 3) return (float)((double)val * 0.35355339059327376220);
+  else if constexpr (K == 4) return val * 0.25f;
+  else return (float)((double)val * 0.17677669529663688110);
+}
+
 // harmonic_sum_kernel (src/kernels.cu:38-98).  Fold k adds the bins
 // (int)(i*j/2^(k+1) + 0.5) for odd j, in the order the reference lists them
 // (fold 1 adds 3/4 before 1/4), to one running float sum, and stores the
@@ -105,27 +116,27 @@ __device__ inline void harmonic_sums(const float* __restrict__ P, uint64_t i, fl
   float val = P[i];
   if (NF > 0) {
     val += P[(i + 1) >> 1];
-    s[0] = (float)((double)val * 0.70710678118654752440);
+    s[0] = fold_scale<1>(val);
   }
   if (NF > 1) {
     val += P[(3 * i + 2) >> 2];
     val += P[(i + 2) >> 2];
-    s[1] = val * 0.5f;
+    s[1] = fold_scale<2>(val);
   }
   if (NF > 2) {
 #pragma unroll
     for (uint64_t j = 1; j < 8; j += 2) val += P[(j * i + 4) >> 3];
-    s[2] = (float)((double)val * 0.35355339059327376220);
+    s[2] = fold_scale<3>(val);
   }
   if (NF > 3) {
 #pragma unroll
     for (uint64_t j = 1; j < 16; j += 2) val += P[(j * i + 8) >> 4];
-    s[3] = val * 0.25f;
+    s[3] = fold_scale<4>(val);
   }
   if (NF > 4) {
 #pragma unroll
     for (uint64_t j = 1; j < 32; j += 2) val += P[(j * i + 16) >> 5];
-    s[4] = (float)((double)val * 0.17677669529663688110);
+    s[4] = fold_scale<5>(val);
   }
 }
 
@@ -136,27 +147,27 @@ __device__ inline void harmonic_sums32(const float* __restrict__ P, uint32_t i, 
   float val = P[i];
   if (NF > 0) {
     val += P[(i + 1) >> 1];
-    s[0] = (float)((double)val * 0.70710678118654752440);
+    s[0] = fold_scale<1>(val);
   }
   if (NF > 1) {
     val += P[(3 * i + 2) >> 2];
     val += P[(i + 2) >> 2];
-    s[1] = val * 0.5f;
+    s[1] = fold_scale<2>(val);
   }
   if (NF > 2) {
 #pragma unroll
     for (uint32_t j = 1; j < 8; j += 2) val += P[(j * i + 4) >> 3];
-    s[2] = (float)((double)val * 0.35355339059327376220);
+    s[2] = fold_scale<3>(val);
   }
   if (NF > 3) {
 #pragma unroll
     for (uint32_t j = 1; j < 16; j += 2) val += P[(j * i + 8) >> 4];
-    s[3] = val * 0.25f;
+    s[3] = fold_scale<4>(val);
   }
   if (NF > 4) {
 #pragma unroll
     for (uint32_t j = 1; j < 32; j += 2) val += P[(j * i + 16) >> 5];
-    s[4] = (float)((double)val * 0.17677669529663688110);
+    s[4] = fold_scale<5>(val);
   }
 }
 
