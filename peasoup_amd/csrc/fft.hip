@@ -69,6 +69,12 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2 (&tw)
           w = tw[b * (R - 1) + r - 1];
         else if constexpr (TWS == TW_LDS)
           w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[k * r * TS];
+        else if constexpr (TWS == TW_LDS_HALF) {
+          const int m = k * r * TS;
+          const bool upper = m >= L / 2;
+          w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[upper ? m - L / 2 : m];
+          if (upper) w = make_float2(-w.x, -w.y);
+        }
         else
           w = ldg2(a.twL, (uint32_t)(k * r * TS));
         v[b * R + r] = cmul(v[b * R + r], w);
@@ -193,10 +199,11 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
   const uint32_t icol = (uint32_t)a.in_col, iel = (uint32_t)a.in_elem;
   float2 v[P];
   float2 tw[P];
-  if constexpr (twiddle_source(L, C, MODE) == TW_LDS) {
+  if constexpr (twiddle_source(L, C, MODE) == TW_LDS || twiddle_source(L, C, MODE) == TW_LDS_HALF) {
     // the first use follows the stage-0 exchange and its barriers
+    constexpr int NTW = twiddle_source(L, C, MODE) == TW_LDS ? L : L / 2;
     float2* ltw = reinterpret_cast<float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4);
-    for (int i = t; i < L; i += NT) ltw[i] = ldg2(a.twL, (uint32_t)i);
+    for (int i = t; i < NTW; i += NT) ltw[i] = ldg2(a.twL, (uint32_t)i);
   }
   if constexpr (RS) {
     // resampleII fused into the first read (src/kernels.cu:342-353); the

@@ -72,22 +72,26 @@ __host__ __device__ constexpr int waves_per_simd(int L, int C, int mode) {
 // after every exchange, so:
 //   TW_LDS       the L-entry table is copied into LDS behind the exchange
 //                buffer when that leaves the workgroups per CU unchanged;
+//   TW_LDS_HALF  else its first L/2 entries, when those fit that way
+//                (W_L^(m + L/2) = -W_L^m: one sign flip per lookup);
 //   TW_PREFETCH  otherwise, when the register budget is at least 128 VGPRs
 //                and a thread holds at most 16 points, each stage's twiddles
 //                are loaded before the preceding LDS exchange (their latency
 //                hides behind it);
 //   TW_GLOBAL    loads at the point of use.
-enum TwiddleSource { TW_GLOBAL = 0, TW_LDS = 1, TW_PREFETCH = 2 };
+enum TwiddleSource { TW_GLOBAL = 0, TW_LDS = 1, TW_PREFETCH = 2, TW_LDS_HALF = 3 };
 __host__ __device__ constexpr int twiddle_source(int L, int C, int mode) {
   const int base = lds_bytes(L, C, split_planes(mode));
   if (num_stages(L) < 2) return TW_GLOBAL;
   if (base + 8 * L <= 163840 && wgs_per_cu(L, C, base + 8 * L) == wgs_per_cu(L, C, base)) return TW_LDS;
+  if (base + 4 * L <= 163840 && wgs_per_cu(L, C, base + 4 * L) == wgs_per_cu(L, C, base)) return TW_LDS_HALF;
   if (points_per_thread(L) <= 16 && 512 / waves_per_simd(L, C, mode) >= 128) return TW_PREFETCH;
   return TW_GLOBAL;
 }
 // Dynamic LDS of a pass kernel: the exchange buffer plus the twiddle table.
 __host__ __device__ constexpr int pass_lds_bytes(int L, int C, int mode) {
-  return lds_bytes(L, C, split_planes(mode)) + (twiddle_source(L, C, mode) == TW_LDS ? 8 * L : 0);
+  return lds_bytes(L, C, split_planes(mode)) +
+         (twiddle_source(L, C, mode) == TW_LDS ? 8 * L : (twiddle_source(L, C, mode) == TW_LDS_HALF ? 4 * L : 0));
 }
 
 // ------------------------------------------------- global memory access
