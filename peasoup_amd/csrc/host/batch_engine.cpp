@@ -40,7 +40,7 @@ uint64_t Engine::device_bytes(uint64_t N, uint32_t B, uint32_t cap) {
   const uint64_t pst = (uint64_t)round_up((int64_t)nb, 64);
   uint64_t b = 0;
   b += 2 * M * B * sizeof(float2);      // FFT work + Z
-  b += pst * B * sizeof(float);         // P
+  b += (pst * B + kHarmWaveBins) * sizeof(float);  // P
   b += 2 * (uint64_t)cap * sizeof(Cand16);
   b += nb * (sizeof(float2) + 2 * sizeof(float)) + nb / 4 * sizeof(float);  // prologue X, A, median
   b += 2 * M * sizeof(float2) + 16 * 1024 * 1024;                          // prologue plans + tables
