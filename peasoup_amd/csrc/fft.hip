@@ -177,7 +177,14 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2 (&tw)
           val = cmul(val, cmul(ldg2(a.tw4_hi, x >> a.tw4_shift), ldg2(a.tw4_lo, x & a.tw4_mask)));
         }
         if (a.out_conj) val.y = -val.y;
-        stg2(out, o0 + (uint32_t)(r * (L / R)) * oel, val);
+        // column passes write whole lines (C*8 bytes per row, 128 for C=16)
+        // and their output is read once by the next pass: streaming stores
+        // (measured 3% faster end to end at 2^23; the 32-byte runs of the
+        // rows-to-columns pass are 2x slower that way)
+        if constexpr (MODE == PM_COLS_TW)
+          stg2_nt(out, o0 + (uint32_t)(r * (L / R)) * oel, val);
+        else
+          stg2(out, o0 + (uint32_t)(r * (L / R)) * oel, val);
       }
     }
   }

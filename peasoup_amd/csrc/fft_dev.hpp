@@ -118,6 +118,15 @@ __device__ inline void stg2(float2* base, uint32_t i, float2 v) {
   w.y = v.y;
   *(PSX_GLOBAL psx_f2v*)((PSX_GLOBAL char*)base + (uint64_t)(i * 8u)) = w;
 }
+// Streaming (non-temporal) store: the line is not kept dirty in the caches
+// for a later write-back that would compete with the next kernel's traffic.
+// Used where whole 128-byte lines are written (column tiles of 16).
+__device__ inline void stg2_nt(float2* base, uint32_t i, float2 v) {
+  psx_f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, (PSX_GLOBAL psx_f2v*)((PSX_GLOBAL char*)base + (uint64_t)(i * 8u)));
+}
 
 // ---------------------------------------------------------- complex math
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
