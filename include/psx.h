@@ -244,7 +244,15 @@ typedef struct psx_batch_opts {
   int32_t fft_npass;       /* 0 = default decomposition */
   int32_t fft_len[3];      /* pass lengths (product = n/2) */
   int32_t fft_cols[3];     /* column tile per pass */
+  int32_t spectrum_layout; /* PSX_SPECTRUM_*: how the last FFT pass hands over the spectra */
 } psx_batch_opts;
+
+/* Spectrum hand-over of the batched path.  AUTO: a two-pass plan whose last
+ * pass has column tiles narrower than 16 writes contiguous rows and the
+ * spectrum pass reads that transposed layout; NATURAL: always natural bin
+ * order; ROWS: the transposed layout, an error when the plan has none.  All
+ * three give identical spectra for the same FFT decomposition. */
+enum { PSX_SPECTRUM_AUTO = 0, PSX_SPECTRUM_NATURAL = 1, PSX_SPECTRUM_ROWS = 2 };
 
 /* Same contract as psx_worker_search, computed by the batched path: all DM
  * prologues, then every (DM, acceleration) trial through fused batches

@@ -94,6 +94,121 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
   }
 }
 
+// The same spectrum pass over the transposed half-length spectrum that a
+// last FFT pass writing contiguous rows leaves: row r (R rows of L
+// elements) holds Z(r, e) = Z[r + R*e].  Bin k = r + R*e pairs with M - k,
+// which is (R - r, L-1-e) for r >= 1 and, for r = 0, (0, L-e), i.e. the
+// virtual row R = row 0 shifted by one column (Z[M] = Z[0]).  Interbinning
+// takes X[k-1] = X(r-1, e) from the row below, in the same column.
+//
+// A workgroup owns lower rows [ra, ra+64) (1 <= r < R/2) and their mirror
+// rows R - r, over 64 columns.  Wave w sweeps pair rows q = r0-1 .. r0+16
+// (r0 = ra + 16w), one column per lane: it loads Z(q, e) (row contiguous)
+// and the mirror Z(R-q, L-1-e) (row descending), untangles both bins with
+// the plain pass's arithmetic, and keeps the previous pair row in registers
+// for the interbin neighbours: P(q, e) uses X(q-1, e), and P(R-q', L-1-e)
+// uses X(R-q'-1, L-1-e), the mirror of pair row q'+1.  Results are
+// transposed through LDS and written as 64-bin runs of P (natural order).
+// Rows 0 and R/2 and bin M pair within their own row: k_spec_edge.
+constexpr int kRowTile = 64;  // lower rows per workgroup (16 per wave)
+constexpr int kColTile = 64;  // columns per workgroup (one per lane)
+
+// X[k] of the pair (Z[k], Z[M-k]) exactly as untangled_bin computes it:
+// the root of min(k, M-k), mirrored for the upper bin.
+__device__ inline void untangle_pair(float2 zk, float2 zm, uint32_t k, uint32_t M, const RootTable& rt, float2& xk,
+                                     float2& xm) {
+  const bool upper = 2 * k > M;
+  const float2 w = root(rt, (uint64_t)(upper ? M - k : k));
+  const float2 wm = make_float2(-w.x, w.y);
+  xk = untangle_one(zk, zm, upper ? wm : w);
+  xm = untangle_one(zm, zk, upper ? w : wm);
+}
+
+__global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
+                                                  uint32_t L, int64_t pstride, RootTable rt,
+                                                  const TrialDesc* __restrict__ descs) {
+  __shared__ float lo_t[kColTile][kRowTile + 1];
+  __shared__ float up_t[kColTile][kRowTile + 1];
+  const uint32_t tr = blockIdx.z;
+  const uint32_t M = R * L, half_rows = R / 2;
+  const float2* y = Y + (uint64_t)tr * M;
+  float* p = P + (int64_t)tr * pstride;
+  const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
+  const uint32_t ra = 1 + blockIdx.y * kRowTile, ea = blockIdx.x * kColTile;
+  const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  const uint32_t e = ea + lane, emir = L - 1 - e;
+  const uint32_t r0 = ra + wv * (kRowTile / 4);
+  // Branch-free sweep (so the loads of later rows issue early): rows past
+  // R/2 are never used and load row R/2 instead; results of rows outside
+  // [1, R/2) go to the spare slot kRowTile of each LDS column.
+  float2 xlo_prev = make_float2(0.f, 0.f), xhi_prev = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < kRowTile / 4 + 2; ++i) {
+    const uint32_t q = min(r0 - 1 + (uint32_t)i, half_rows);
+    const uint32_t ih = q == 0 ? (e == 0 ? 0 : L - e) : (R - q) * L + emir;
+    const float2 zl = y[q * L + e];
+    const float2 zh = y[ih];
+    float2 xlo, xhi;
+    untangle_pair(zl, zh, q + R * e, M, rt, xlo, xhi);
+    const uint32_t qq = r0 - 1 + (uint32_t)i;
+    // lower bin (q, e) for q in [r0, r0+16)
+    if (i >= 1 && i <= kRowTile / 4)
+      lo_t[lane][qq < half_rows ? qq - ra : kRowTile] = normalised(xlo, xlo_prev, mean_n, std_n);
+    // upper bin (R-q', L-1-e) for q' = q-1 in [r0, r0+16)
+    if (i >= 2) up_t[lane][qq - 1 < half_rows ? qq - 1 - ra : kRowTile] = normalised(xhi_prev, xhi, mean_n, std_n);
+    xlo_prev = xlo;
+    xhi_prev = xhi;
+  }
+  __syncthreads();
+  const uint32_t rr = threadIdx.x & 63u;
+  const uint32_t r = ra + rr;
+  if (r < half_rows) {
+#pragma unroll 4
+    for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
+      p[r + R * (ea + j)] = lo_t[j][rr];
+      p[(R - r) + R * (L - 1 - (ea + j))] = up_t[j][rr];
+    }
+  }
+}
+
+// Bins of rows 0 and R/2 (k = R*e and R/2 + R*e) and the Nyquist bin M of
+// the transposed layout, one thread each, through the natural-index view.
+__device__ inline float2 z_rows(const float2* y, uint32_t R, uint32_t L, uint32_t j) {
+  const uint32_t M = R * L;
+  if (j >= M) j -= M;
+  return y[(j % R) * L + j / R];
+}
+__device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_t k, const RootTable& rt) {
+  const uint32_t M = R * L;
+  float2 xk, xm;
+  untangle_pair(z_rows(y, R, L, k), z_rows(y, R, L, M - k), k, M, rt, xk, xm);
+  return xk;
+}
+
+__global__ void __launch_bounds__(kB) k_spec_edge(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
+                                                  uint32_t L, int64_t pstride, RootTable rt,
+                                                  const TrialDesc* __restrict__ descs) {
+  const uint32_t tr = blockIdx.y;
+  const uint32_t M = R * L;
+  const float2* y = Y + (uint64_t)tr * M;
+  float* p = P + (int64_t)tr * pstride;
+  const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
+  for (uint32_t t = blockIdx.x * kB + threadIdx.x; t <= 2 * L; t += gridDim.x * kB) {
+    // t < L: row 0; t < 2L: row R/2; t == 2L: bin M
+    const uint32_t k = t < L ? R * t : (t < 2 * L ? R / 2 + R * (t - L) : M);
+    float2 x, prev = make_float2(0.f, 0.f);
+    if (k == M) {
+      // X[M] is the upper bin of pair 0
+      float2 x0;
+      untangle_pair(y[0], y[0], 0, M, rt, x0, x);
+    } else {
+      x = x_rows(y, R, L, k, rt);
+    }
+    if (k > 0) prev = x_rows(y, R, L, k - 1, rt);
+    p[k] = normalised(x, prev, mean_n, std_n);
+  }
+}
+
 // harmonic_sum_kernel (src/kernels.cu:34-100) + device_find_peaks
 // (src/kernels.cu:391-423) for the fundamental and every fold, each over
 // its own PeakFinder window (peakfinder.hpp:78-83).  Survivors are appended
@@ -223,6 +338,20 @@ int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, con
   const unsigned gx = psx::grid_for(chunks * 64, kB * Q, 2048);
   k_spec_post<Q><<<dim3(gx, ntr), kB, 0, s>>>(Z, P, M, pstride, rt, d_descs);
   PSX_CHECK_LAUNCH("k_spec_post");
+  return PSX_OK;
+}
+
+int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t pstride, const RootTable& rt,
+                     const TrialDesc* d_descs, uint32_t ntr, hipStream_t s) {
+  if (R < 4 || R % 2 || L % kColTile || (uint64_t)R * L > 0x7FFFFFFFull || pstride < (int64_t)R * L + 1)
+    return psx::fail(PSX_ERR_ARG, "spec_rows: unsupported layout %u x %u", R, L);
+  const unsigned row_tiles = (R / 2 - 1 + kRowTile - 1) / kRowTile;
+  if (row_tiles > 0) {
+    k_spec_rows<<<dim3(L / kColTile, row_tiles, ntr), kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+    PSX_CHECK_LAUNCH("k_spec_rows");
+  }
+  k_spec_edge<<<dim3(psx::grid_for(2 * (uint64_t)L + 1, kB, 64), ntr), kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+  PSX_CHECK_LAUNCH("k_spec_edge");
   return PSX_OK;
 }
 

@@ -29,6 +29,10 @@ struct Windows {
 
 int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, const psx::fft::RootTable& rt,
                      const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s);
+// Spectrum pass over the transposed layout of a row-writing last FFT pass
+// (row r of L elements holds Z[r + R*e]); same output as launch_spec_post.
+int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t pstride,
+                     const psx::fft::RootTable& rt, const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s);
 // Bins per wave of the harmonic pass.  It reads up to kHarmWaveBins - 1
 // floats past the window end of a spectrum row, so a spectrum buffer of B
 // rows of pstride floats is allocated with kHarmWaveBins floats of slack.

@@ -318,6 +318,8 @@ constexpr KernelInfo kinfo1() {
 }
 // single-pass kernels (C = 1): contiguous, plain and resampling
 #define PSX_K1(L) kinfo1<L, 1, PM_CONTIG, false>(), kinfo1<L, 1, PM_CONTIG, true>()
+// contiguous rows in and out with column tiles (transposed last passes)
+#define PSX_KR(L, C) kinfo1<L, C, PM_CONTIG, false>()
 // column-tile kernels: strided in/out with twiddle (plain and resampling)
 // and rows-to-columns
 #define PSX_KC(L, C) \
@@ -334,6 +336,9 @@ const KernelInfo kKernels[] = {
     PSX_KC(48, 16), PSX_KC(96, 16), PSX_KC(192, 16), PSX_KC(384, 16), PSX_KC(768, 16),
     PSX_KC(96, 8), PSX_KC(192, 8), PSX_KC(384, 8), PSX_KC(768, 8), PSX_KC(1536, 8),
     PSX_KC(384, 4), PSX_KC(768, 4), PSX_KC(1536, 4), PSX_KC(3072, 4),
+    PSX_KR(512, 8), PSX_KR(1024, 8), PSX_KR(2048, 8), PSX_KR(768, 8), PSX_KR(1536, 8),
+    PSX_KR(1024, 4), PSX_KR(2048, 4), PSX_KR(4096, 4), PSX_KR(1536, 4), PSX_KR(3072, 4),
+    PSX_KR(2048, 2), PSX_KR(4096, 2),
 };
 
 const KernelInfo* find_kernel(int L, int C, int mode = -1, bool rs = false) {
@@ -509,6 +514,11 @@ struct psx_fft_plan {
   uint32_t batch = 1;
   psx::fft::Decomp d;
   std::vector<PassLaunch> passes;
+  // Two-pass R2C plans whose last pass writes narrow column runs (C < 16)
+  // also carry a version of that pass writing contiguous rows: Y[r*M1 + e]
+  // = Z[r + M2*e], the transposed layout the batched spectrum pass reads.
+  PassLaunch rows_last{};
+  bool has_rows_last = false;
   float2* tables = nullptr;  // all device tables in one allocation
   float2* work = nullptr;    // M complex per transform (when needed)
   RootTable rt{};            // W_n roots for (un)tangle
@@ -605,6 +615,17 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
     B.gx = (unsigned)(M2 / d.C[1]); B.gy = 1; B.gz = p->batch;
     p->passes.push_back(A);
     p->passes.push_back(B);
+    if (!c2r && d.C[1] < 16) {
+      PassLaunch Bt = make(1, 1, 2, PM_CONTIG);
+      Bt.a.in_col = M1; Bt.a.in_elem = 1; Bt.a.in_sub = 0;
+      Bt.a.out_col = M1; Bt.a.out_elem = 1; Bt.a.out_sub = 0;
+      Bt.a.in_cf = 0; Bt.a.out_cf = 0;
+      Bt.gx = B.gx; Bt.gy = 1; Bt.gz = p->batch;
+      if (Bt.k) {
+        p->rows_last = Bt;
+        p->has_rows_last = true;
+      }
+    }
   } else {
     const int64_t Q = d.L[0], M2 = d.L[1], M1 = d.L[2], P = M1 * M2;
     PassLaunch A = make(0, c2r ? 1 : 0, 1, PM_COLS_TW);
@@ -740,10 +761,14 @@ int check_pass_bounds(const PassLaunch& L, const PassArgs& a, uint32_t gz, uint6
   return PSX_OK;
 }
 
-int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs) {
+int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs,
+                    bool rows_out) {
   if (p->kind != PSX_FFT_R2C || ntrials < 1 || ntrials > p->batch || q < 0 || q >= (int)p->passes.size())
     return psx::fail(PSX_ERR_ARG, "exec_fused_pass: bad plan, pass or trial count");
-  PassLaunch& L = p->passes[q];
+  const bool last = q == (int)p->passes.size() - 1;
+  if (rows_out && last && !p->has_rows_last)
+    return psx::fail(PSX_ERR_UNSUPPORTED, "exec_fused_pass: plan has no row-layout last pass");
+  PassLaunch& L = (rows_out && last) ? p->rows_last : p->passes[q];
   PassArgs a = L.a;
   a.in = L.src == 1 ? p->work : nullptr;
   a.out = L.dst == 2 ? zout : p->work;
@@ -765,12 +790,20 @@ int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t n
   return PSX_OK;
 }
 
-int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs) {
+int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t hs,
+                   bool rows_out) {
   for (int q = 0; q < (int)p->passes.size(); ++q) {
-    int rc = exec_fused_pass(p, q, d_descs, ntrials, zout, hs);
+    int rc = exec_fused_pass(p, q, d_descs, ntrials, zout, hs, rows_out);
     if (rc) return rc;
   }
   return PSX_OK;
+}
+
+bool rows_layout(const psx_fft_plan* p, uint32_t* rows, uint32_t* row_len) {
+  if (!p->has_rows_last) return false;
+  *rows = (uint32_t)p->d.L[0];
+  *row_len = (uint32_t)p->d.L[1];
+  return true;
 }
 
 int num_passes(const psx_fft_plan* p) { return (int)p->passes.size(); }

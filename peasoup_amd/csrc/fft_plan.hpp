@@ -56,8 +56,16 @@ int plan_exec(psx_fft_plan* p, const void* in, void* out, hipStream_t s);
 // Forward complex passes of an R2C plan whose first pass reads trial
 // blockIdx.z through the resampler (descs); writes the half-length complex
 // spectra Z (trial stride M) to zout without untangling.
-int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s);
-int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s);
+// rows_out: the last pass writes the transposed layout instead (see
+// rows_layout); only for plans that have it.
+int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s,
+                   bool rows_out = false);
+int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s,
+                    bool rows_out = false);
+// True when a two-pass plan's last pass can write contiguous rows: row r of
+// row_len elements then holds Z[r + rows*e], e = 0..row_len-1 (rows = the
+// first pass's length), instead of the natural Z[k].
+bool rows_layout(const psx_fft_plan* p, uint32_t* rows, uint32_t* row_len);
 int num_passes(const psx_fft_plan* p);
 RootTable root_table(const psx_fft_plan* p);
 uint64_t complex_len(const psx_fft_plan* p);

@@ -55,6 +55,13 @@ Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t s
   pstride = round_up((int64_t)nb, 64);
   psx_ok(psx::fft::plan_create_with(N, PSX_FFT_R2C, B, opt.decomp, &plan), "batched R2C plan");
   rt = psx::fft::root_table(plan);
+  if (opt.spectrum_layout < PSX_SPECTRUM_AUTO || opt.spectrum_layout > PSX_SPECTRUM_ROWS)
+    throw std::runtime_error("spectrum_layout must be PSX_SPECTRUM_AUTO, _NATURAL or _ROWS");
+  const bool rows_ok =
+      psx::fft::rows_layout(plan, &rows_R, &rows_L) && rows_L % 64 == 0 && rows_R % 2 == 0 && rows_R >= 4;
+  if (opt.spectrum_layout == PSX_SPECTRUM_ROWS && !rows_ok)
+    throw std::runtime_error("spectrum_layout ROWS: this FFT plan has no row-layout last pass");
+  rows_out = rows_ok && opt.spectrum_layout != PSX_SPECTRUM_NATURAL;
   dalloc(&Z, M * B);
   dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
   for (Slot& sl : slots) {
@@ -142,13 +149,23 @@ void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, floa
   psx_ok(psx_fft_exec(p_c2r, X, d_tim, s), "prologue C2R");
 }
 
+// Spectrum pass: from the natural half-length spectra, or from the
+// transposed rows when the plan's last pass writes those (its 32/64-byte
+// column runs cost more than the spectrum pass's tiled transpose).
+void Engine::spectrum(Slot& sl, uint32_t n) {
+  if (rows_out)
+    psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s), "spec_rows");
+  else
+    psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+}
+
 void Engine::enqueue(Slot& sl, uint32_t n) {
   auto record = [&]() {
     hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s),
            "descriptor upload");
     psx_ok(launch_zero(sl.d_count, s), "zero");
-    psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s), "fused FFT");
-    psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+    psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
+    spectrum(sl, n);
     psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
     hip_ok(hipMemcpyAsync(sl.h_count, sl.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count copy");
     hip_ok(hipMemcpyAsync(sl.h_cands, sl.d_cands, kPrefix * sizeof(Cand16), hipMemcpyDeviceToHost, s),
@@ -305,10 +322,10 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
     psx_ok(launch_zero(sl.d_count, s), "zero");
     hip_ok(hipEventRecord(ev[0], s), "event");
     for (int q = 0; q < np; ++q) {
-      psx_ok(psx::fft::exec_fused_pass(plan, q, sl.d_desc, n, Z, s), "fused pass");
+      psx_ok(psx::fft::exec_fused_pass(plan, q, sl.d_desc, n, Z, s, rows_out), "fused pass");
       hip_ok(hipEventRecord(ev[q + 1], s), "event");
     }
-    psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+    spectrum(sl, n);
     hip_ok(hipEventRecord(ev[np + 1], s), "event");
     psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
     hip_ok(hipEventRecord(ev[np + 2], s), "event");
@@ -328,6 +345,7 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
   out.desc.clear();
   for (int q = 0; q < d.npass; ++q)
     out.desc += (q ? "x" : "") + std::to_string(d.L[q]) + "/C" + std::to_string(d.C[q]);
+  if (rows_out) out.desc += " rows";
 }
 
 }  // namespace batch

@@ -37,6 +37,7 @@ struct EngineOptions {
   uint32_t cand_capacity = 1u << 20;
   bool use_graph = true;
   const psx::fft::Decomp* decomp = nullptr;  // FFT decomposition override (autotuner)
+  int spectrum_layout = PSX_SPECTRUM_AUTO;
 };
 
 struct StageTimes {
@@ -92,6 +93,10 @@ class Engine {
     std::map<uint32_t, hipGraphExec_t> graphs;
   };
   void enqueue(Slot& sl, uint32_t n);
+  void spectrum(Slot& sl, uint32_t n);
+  // the fused FFT leaves the transposed row layout (rows_R rows of rows_L)
+  bool rows_out = false;
+  uint32_t rows_R = 0, rows_L = 0;
   bool collect_oldest(std::vector<std::vector<Candidate>>& per_trial);
   void process(Slot& sl, std::vector<std::vector<Candidate>>& per_trial);
 

@@ -82,6 +82,7 @@ bool engine_options(const psx_batch_opts* opts, EngineOptions& eo, psx::fft::Dec
     eo.batch = opts->batch ? opts->batch : eo.batch;
     eo.cand_capacity = opts->cand_capacity ? opts->cand_capacity : eo.cand_capacity;
     eo.use_graph = opts->use_graph != 0;
+    eo.spectrum_layout = opts->spectrum_layout;
     if (opts->fft_npass > 0) {
       forced.npass = opts->fft_npass;
       for (int q = 0; q < 3; ++q) {

@@ -153,6 +153,81 @@ def test_batched_uneven_sizes_match_worker(psx, n):
     assert same(got, ref)
 
 
+# Two-pass plans whose last pass has narrow column tiles write the spectrum
+# as contiguous rows (R = first-pass length rows of L) and the spectrum pass
+# reads that transposed layout: tile interiors, the halo rows, the self-paired
+# rows 0 and R/2, partial row tiles (R/2 - 1 not a multiple of 64) and
+# radix-3 lengths.  For the same FFT decomposition the candidates must equal
+# those of the natural layout exactly (a different decomposition, as the
+# plain Worker's default can be, rounds the FFT differently).
+def _plan_desc(psx, W, w, series0, accs, opts):
+    import ctypes as C
+    stage_ms = np.zeros(8)
+    ns = C.c_int(8)
+    desc = C.create_string_buffer(128)
+    d = psx.DeviceArray.from_numpy(np.ascontiguousarray(series0, np.float32))
+    psx.register_signatures({"psx_batch_profile": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
+                                                             C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
+                                                             C.c_int, C.c_void_p, C.POINTER(C.c_int), C.c_char_p,
+                                                             C.c_size_t])})
+    acc32 = np.ascontiguousarray(accs, np.float32)
+    psx.check(psx.lib().psx_batch_profile(C.byref(w.search_args()), C.byref(opts), d.ptr, w.size,
+                                          acc32.ctypes.data, acc32.size, 1, stage_ms.ctypes.data, C.byref(ns),
+                                          desc, 128))
+    return desc.value.decode()
+
+
+@pytest.mark.parametrize("n,decomp", [
+    (1 << 17, [(32, 16), (2048, 8)]),         # R = 32: one partial row tile
+    (1 << 20, [(256, 16), (2048, 4)]),        # R = 256: two row tiles
+    (3 << 19, [(384, 16), (2048, 8)]),        # R = 384: R/2 - 1 = 191, last tile partial
+    (3 << 19, [(512, 16), (1536, 8)]),        # radix-3 row length
+    (1 << 23, [(1024, 16), (4096, 4)]),       # the 2^23 workload's plan
+])
+def test_row_layout_equals_natural_layout(psx, n, decomp):
+    import peasoup_amd.workloads as W
+    w = W.Workload("rows", 2, n, 64e-6, 0x5EA5C0 + n % 97, 0.0, -50.0, 50.0,
+                   injections=[W.Injection(1, 0.0123, 20.0, 15.0, 0.2)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    accs = accs[:: max(1, accs.size // 12)]
+    rows = W.make_batch_opts(4, decomp=decomp, spectrum_layout=W.SPECTRUM_ROWS)
+    nat = W.make_batch_opts(4, decomp=decomp, spectrum_layout=W.SPECTRUM_NATURAL)
+    assert _plan_desc(psx, W, w, series[0], accs, rows).endswith(" rows")
+    assert not _plan_desc(psx, W, w, series[0], accs, nat).endswith(" rows")
+    ref, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, nat, forest=True)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, rows, forest=True)
+    assert len(ref) > 0
+    assert same(got, ref)
+
+
+def test_row_layout_default_plan_matches_worker(psx):
+    """The default 2^21 plan takes the row layout automatically and equals
+    the plain Worker (same decomposition) bit for bit."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("rows", 2, 1 << 21, 64e-6, 0x5EA5C7, 0.0, -50.0, 50.0,
+                   injections=[W.Injection(0, 0.0077, 18.0, 12.0, 0.2)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    accs = accs[:: max(1, accs.size // 10)]
+    opts = W.make_batch_opts(4)
+    assert _plan_desc(psx, W, w, series[0], accs, opts).endswith(" rows")
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs, forest=True)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, opts, forest=True)
+    assert len(ref) > 0
+    assert same(got, ref)
+
+
+def test_row_layout_unavailable_is_an_error(psx):
+    import peasoup_amd.workloads as W
+    w = W.Workload("rows", 1, 1 << 17, 64e-6, 0x5EA5C8, 0.0, -5.0, 5.0)
+    series = W.generate_host(w)
+    # a three-pass plan has no row-layout last pass
+    opts = W.make_batch_opts(2, decomp=[(16 * 2, 16), (32, 16), (64, 16)], spectrum_layout=W.SPECTRUM_ROWS)
+    with pytest.raises(psx.PsxError, match="row-layout"):
+        W.batch_search(w.search_args(), series, w.dm_list(), None, opts)
+
+
 def test_batched_overflow_is_an_error(psx):
     """A threshold below the noise floor floods the candidate buffer: the
     search must fail loudly rather than drop candidates."""
