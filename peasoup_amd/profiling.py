@@ -88,7 +88,9 @@ def stage_of(name):
     """Batched-stage label of a fused kernel name (or None)."""
     if "k_fft_pass<" in name:
         return "fft_pass0_resample" if re.search(r",\s*true>\s*$", name) else "fft_pass"
-    if "k_spec_post" in name:
+    # the spectrum pass: natural layout (k_spec_post) or the transposed row
+    # layout (k_spec_rows; its small companion k_spec_edge is not counted)
+    if "k_spec_post" in name or "k_spec_rows" in name:
         return "spec_post"
     if "k_harm_peaks" in name:
         return "harm_peaks"
