@@ -95,7 +95,7 @@ def candidates(n, limit=12):
     return out
 
 
-def tune(n, tsamp, d_series, accs, batches=(8, 16, 32), reps=5, limit=12, log=print):
+def tune(n, tsamp, d_series, accs, batches=(16, 32, 64), reps=5, limit=12, log=print):
     """Times candidate schedules on one prologued series; returns the best."""
     import peasoup_amd as P
     import peasoup_amd.workloads as W
@@ -137,7 +137,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--sizes", default="8388608")
     ap.add_argument("--tsamp", type=float, default=76e-6)
-    ap.add_argument("--batches", default="8,16,32")
+    ap.add_argument("--batches", default="16,32,64")
     ap.add_argument("--limit", type=int, default=12)
     ap.add_argument("--out", default=CACHE)
     a = ap.parse_args(argv)
