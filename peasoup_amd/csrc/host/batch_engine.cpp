@@ -100,6 +100,8 @@ Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t s
   dalloc(&med, nb);
   dalloc(&med_work, psx_deredden_work_floats(nb));
   dalloc((uint8_t**)&red_work, psx_reduce_work_bytes());
+  dalloc(&d_base, 1);
+  dalloc(&d_sums, 1);
 }
 
 Engine::~Engine() {
@@ -124,6 +126,8 @@ Engine::~Engine() {
   (void)hipFree(med);
   (void)hipFree(med_work);
   (void)hipFree(red_work);
+  (void)hipFree(d_base);
+  (void)hipFree(d_sums);
 }
 
 void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std,
@@ -157,6 +161,37 @@ void Engine::spectrum(Slot& sl, uint32_t n) {
     psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s), "spec_rows");
   else
     psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+}
+
+void Engine::prologue_async(const float* d_series, uint64_t n_real, float* d_tim, PrologueResult* res,
+                            hipStream_t ps, const Zapper* zap) {
+  // prologue() step for step, on ps
+  const uint64_t ncopy = std::min<uint64_t>(n_real, N);
+  if (d_series != d_tim)
+    hip_ok(hipMemcpyAsync(d_tim, d_series, ncopy * sizeof(float), hipMemcpyDeviceToDevice, ps), "prologue copy");
+  psx_ok(psx::remove_baseline_async(d_tim, ncopy, red_work, d_base, kBaselineSteps, ps), "remove_baseline");
+  if (N > n_real) psx_ok(psx_fill(d_tim, n_real, N, 0.0f, ps), "fill");
+  const float tobs = (unsigned int)N * args.tsamp;
+  const float bin_width = 1.0 / tobs;
+  psx_ok(psx_fft_exec(p_r2c, d_tim, X, ps), "prologue R2C");
+  psx_ok(psx_form_amplitude(X, A, nb, ps), "form");
+  psx_ok(psx_deredden_median(A, nb, (double)bin_width, 0.05f, 0.5f, med, med_work, ps), "calculate_median");
+  psx_ok(psx_divide_c_by_f(X, med, nb, ps), "deredden");
+  if (zap) zap->zap_on(X, bin_width, nb, ps);
+  psx_ok(psx_form_interbin(X, A, nb, ps), "form_interpolated");
+  psx_ok(psx::sum_sq_async(A, 0, nb, red_work, d_sums, ps), "stats");
+  hip_ok(hipMemcpyAsync(&res->base, d_base, sizeof(BaselineState), hipMemcpyDeviceToHost, ps), "baseline state");
+  hip_ok(hipMemcpyAsync(&res->sums, d_sums, sizeof(double2), hipMemcpyDeviceToHost, ps), "stats copy");
+  psx_ok(psx_fft_exec(p_c2r, X, d_tim, ps), "prologue C2R");
+}
+
+// psx_mean_rms's arithmetic on the sums, then prologue()'s std.
+void Engine::stats_of(const PrologueResult& res, float* mean, float* std) const {
+  const float cnt = (float)(int64_t)nb;
+  const float m = (float)res.sums.x / cnt;
+  const float r = sqrtf((float)res.sums.y / cnt);
+  *mean = m;
+  *std = std::sqrt(r * r - m * m);
 }
 
 void Engine::enqueue(Slot& sl, uint32_t n) {

@@ -16,6 +16,7 @@
 
 #include "batch.hpp"
 #include "fft_plan.hpp"
+#include "reduce_async.hpp"
 #include "peasoup/candidates.hpp"
 #include "peasoup/transforms.hpp"
 #include "psx.h"
@@ -30,6 +31,13 @@ struct TrialWork {
   float std;
   float dm;
   int dm_idx;
+};
+
+// Results of an asynchronous prologue, written by the device into pinned
+// host memory: the baseline loop's state and the interbinned spectrum's sums.
+struct PrologueResult {
+  BaselineState base;
+  double2 sums;  // (sum x, sum x^2) over the nb bins
 };
 
 struct EngineOptions {
@@ -72,6 +80,17 @@ class Engine {
   // zap (optional) removes birdies after dereddening (pipeline_multi.cu:222-226).
   void prologue(const float* d_series, uint64_t n_real, float* d_tim, float* mean, float* std,
                 const Zapper* zap = nullptr);
+  // The same prologue enqueued on stream ps with no host round trip: the
+  // baseline loop runs kBaselineSteps device iterations and the statistics
+  // arrive in *res (pinned host memory) when ps reaches them.  Consecutive
+  // calls must share ps (they reuse the prologue buffers in order).  If
+  // res->base.done is 0 once ps has passed the call, the baseline loop had
+  // not stopped: redo the DM with prologue().  stats_of turns *res into the
+  // (mean, std) prologue() returns.
+  static constexpr int kBaselineSteps = 4;
+  void prologue_async(const float* d_series, uint64_t n_real, float* d_tim, PrologueResult* res, hipStream_t ps,
+                      const Zapper* zap = nullptr);
+  void stats_of(const PrologueResult& res, float* mean, float* std) const;
 
   // Average duration of each launch of one batch of n trials, timed with HIP
   // events on the engine stream over `reps` repetitions (after one warm-up).
@@ -125,6 +144,8 @@ class Engine {
   float* med = nullptr;
   float* med_work = nullptr;
   void* red_work = nullptr;
+  BaselineState* d_base = nullptr;
+  double2* d_sums = nullptr;
 };
 
 }  // namespace batch

@@ -1,13 +1,17 @@
 // batch_search.cpp - whole-search driver over the batched engine: the
-// Worker's DM loop (src/pipeline_multi.cu:155-295) restructured so that all
-// DM prologues run first and the acceleration trials of every DM stream
-// through fused, graph-launched batches.  Candidate semantics are the
+// Worker's DM loop (src/pipeline_multi.cu:155-295) restructured so that the
+// DM prologues run ahead on a stream of their own and the acceleration
+// trials of every DM stream through fused, graph-launched batches as soon as
+// their DM's prologue is done.  Candidate semantics are the
 // Worker's: per-trial HarmonicDistiller, per-DM AccelerationDistiller over
 // the trials in acceleration-list order.
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "batch_engine.hpp"
@@ -22,6 +26,9 @@ using psx::batch::TrialWork;
 namespace {
 using clk = std::chrono::steady_clock;
 double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
 }  // namespace
 
 namespace psx {
@@ -32,7 +39,7 @@ namespace search {
 void run_trials(Engine& eng, const psx_search_args& a, const std::vector<const float*>& tims,
                 const std::vector<float>& means, const std::vector<float>& stds, const std::vector<float>& dms,
                 const std::vector<int>& dm_idx, const std::vector<std::vector<float>>& accs,
-                std::vector<std::vector<Candidate>>& per_dm) {
+                std::vector<std::vector<Candidate>>& per_dm, const std::function<void(size_t)>& ready) {
   const size_t ndm = tims.size();
   std::vector<TrialWork> all;
   std::vector<std::pair<size_t, size_t>> where;  // (dm, acc index) of each trial
@@ -47,9 +54,21 @@ void run_trials(Engine& eng, const psx_search_args& a, const std::vector<const f
   size_t next = 0, done = 0;
   std::vector<size_t> starts;  // first trial of each submitted batch
   std::vector<std::vector<Candidate>> got;
+  std::vector<char> is_ready(ndm, ready ? 0 : 1);
   while (done < all.size()) {
     while (next < all.size() && eng.in_flight() < 2) {
       const uint32_t n = (uint32_t)std::min<size_t>(B, all.size() - next);
+      // a DM's statistics (and its series on the device) exist once its
+      // prologue has finished: ready(d) waits for that
+      for (size_t t = next; t < next + n; ++t) {
+        const size_t d = where[t].first;
+        if (!is_ready[d]) {
+          ready(d);
+          is_ready[d] = 1;
+        }
+        all[t].mean = means[d];
+        all[t].std = stds[d];
+      }
       eng.submit(&all[next], n);
       starts.push_back(next);
       next += n;
@@ -146,16 +165,36 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
   if (rc) return rc;
   PSX_REQUIRE(series && dm_list && ndm > 0 && nsamps > 0, "psx_batch_search: bad series");
   PSX_REQUIRE(!accs || naccs > 0, "psx_batch_search: empty acceleration list");
-  hipStream_t s = nullptr;
+  hipStream_t s = nullptr, ps = nullptr;
   float* d_tims = nullptr;
   float* d_stage = nullptr;
+  psx::batch::PrologueResult* h_res = nullptr;
+  std::vector<hipEvent_t> done_ev;
+  auto release = [&]() {
+    if (ps) (void)hipStreamSynchronize(ps);
+    for (auto& e : done_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (h_res) (void)hipHostFree(h_res);
+    if (d_tims) (void)hipFree(d_tims);
+    if (d_stage) (void)hipFree(d_stage);
+    if (ps) (void)hipStreamDestroy(ps);
+    if (s) (void)hipStreamDestroy(s);
+  };
   try {
     EngineOptions eo;
     psx::fft::Decomp forced;
     engine_options(opts, eo, forced);
     PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    PSX_CHECK_HIP(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
     const uint64_t N = args->size;
     std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
+    // declared after the engine, so destroyed first on every exit path: no
+    // prologue work may still be queued on ps when the engine frees the
+    // buffers it uses (a DM without trials is never waited for)
+    struct DrainGuard {
+      hipStream_t st;
+      ~DrainGuard() { (void)hipStreamSynchronize(st); }
+    } drain{ps};
     PSX_CHECK_HIP(hipMalloc(&d_tims, sizeof(float) * N * ndm));
     const uint64_t nreal = nsamps;
     auto t0 = clk::now();
@@ -167,15 +206,22 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
                           args->tsamp, args->cfreq, args->ch_bw);
     if (!series_on_device) PSX_CHECK_HIP(hipMalloc(&d_stage, sizeof(float) * nreal));
     std::unique_ptr<Zapper> zap;
-    if (args->nbirdies) zap.reset(new Zapper(args->birdie_freq, args->birdie_width, args->nbirdies, s));
+    if (args->nbirdies) zap.reset(new Zapper(args->birdie_freq, args->birdie_width, args->nbirdies, ps));
+    PSX_CHECK_HIP(hipHostMalloc((void**)&h_res, sizeof(psx::batch::PrologueResult) * ndm, hipHostMallocDefault));
+    done_ev.assign(ndm, nullptr);
+    for (auto& e : done_ev) PSX_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // every prologue is enqueued at once on ps; the trials of DM d wait for
+    // its event only
+    auto input_of = [&](uint32_t d, hipStream_t st) {
+      const float* src = series + (uint64_t)d * nreal;
+      if (series_on_device) return src;
+      hip_check(hipMemcpyAsync(d_stage, src, sizeof(float) * nreal, hipMemcpyHostToDevice, st), "series upload");
+      return (const float*)d_stage;
+    };
     for (uint32_t d = 0; d < ndm; ++d) {
       float* tim = d_tims + (uint64_t)d * N;
-      const float* src = series + (uint64_t)d * nreal;
-      if (!series_on_device) {
-        PSX_CHECK_HIP(hipMemcpyAsync(d_stage, src, sizeof(float) * nreal, hipMemcpyHostToDevice, s));
-        src = d_stage;
-      }
-      eng->prologue(src, nreal, tim, &means[d], &stds[d], zap.get());
+      eng->prologue_async(input_of(d, ps), nreal, tim, &h_res[d], ps, zap.get());
+      PSX_CHECK_HIP(hipEventRecord(done_ev[d], ps));
       tims[d] = tim;
       idx[d] = (int)d;
       if (accs)
@@ -183,10 +229,24 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
       else
         plan.generate_accel_list(dms[d], args->cdm, acc_lists[d]);
     }
-    PSX_CHECK_HIP(hipStreamSynchronize(s));
+    auto ready = [&](size_t d) {
+      hip_check(hipEventSynchronize(done_ev[d]), "prologue event");
+      if (!h_res[d].base.done) {
+        // the baseline loop needs more than the device steps: redo this DM
+        // with the host-driven prologue once ps is idle (its buffers are
+        // shared), from its original input
+        hip_check(hipStreamSynchronize(ps), "prologue stream");
+        eng->prologue(input_of((uint32_t)d, s), nreal, const_cast<float*>(tims[d]), &means[d], &stds[d],
+                      zap.get());
+        hip_check(hipStreamSynchronize(s), "prologue");
+      } else {
+        eng->stats_of(h_res[d], &means[d], &stds[d]);
+      }
+      hip_check(hipStreamWaitEvent(s, done_ev[d], 0), "stream wait");
+    };
     auto t1 = clk::now();
     std::vector<std::vector<Candidate>> per_dm;
-    psx::search::run_trials(*eng, *args, tims, means, stds, dms, idx, acc_lists, per_dm);
+    psx::search::run_trials(*eng, *args, tims, means, stds, dms, idx, acc_lists, per_dm, ready);
     auto t2 = clk::now();
     std::vector<Candidate> flat;
     for (auto& v : per_dm) flat.insert(flat.end(), v.begin(), v.end());
@@ -195,15 +255,12 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
       seconds[1] = secs(t1, t2);
       seconds[2] = secs(t0, t2);
     }
+    hip_check(hipStreamSynchronize(ps), "prologue stream");
     eng.reset();
-    (void)hipFree(d_tims);
-    if (d_stage) (void)hipFree(d_stage);
-    (void)hipStreamDestroy(s);
+    release();
     return psx::search::flatten(flat, out, cap, count) == PSX_OK ? psx::ok() : PSX_ERR_OVERFLOW;
   } catch (const std::exception& e) {
-    if (d_tims) (void)hipFree(d_tims);
-    if (d_stage) (void)hipFree(d_stage);
-    if (s) (void)hipStreamDestroy(s);
+    release();
     return psx::fail(PSX_ERR_HIP, "psx_batch_search: %s", e.what());
   }
 }

@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "kernels_dev.hpp"
+#include "reduce_async.hpp"
 
 using psx::as_stream;
 
@@ -229,6 +230,33 @@ __global__ void __launch_bounds__(kBlock) k_normalise(float* __restrict__ p, uin
 __global__ void __launch_bounds__(kBlock) k_subtract(float* __restrict__ x, uint64_t n, float mean) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] -= mean;
+}
+
+// Device-resident remove_baseline loop (the host loop of
+// psx_remove_baseline with its arithmetic): each step subtracts the mean of
+// the sums in `sums` unless the loop has already stopped, then records
+// whether it continues.
+__global__ void __launch_bounds__(kBlock) k_subtract_dev(float* __restrict__ x, uint64_t n,
+                                                         const double2* __restrict__ sums,
+                                                         const psx::BaselineState* __restrict__ st) {
+  if (st->done) return;
+  const float mean = (float)sums->x / (float)(int64_t)n;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] -= mean;
+}
+__global__ void k_baseline_step(uint64_t n, const double2* __restrict__ sums, psx::BaselineState* __restrict__ st) {
+  if (blockIdx.x != 0 || threadIdx.x != 0 || st->done) return;
+  const float mean = (float)sums->x / (float)(int64_t)n;
+  // do { ...; if (count++ > 100) break; } while (fabsf(mean) > 5e-7 * (double)n);
+  const int32_t count = st->count;
+  st->count = count + 1;
+  st->done = (count > 100 || !((double)fabsf(mean) > 5e-7 * (double)n)) ? 1 : 0;
+}
+__global__ void k_baseline_init(psx::BaselineState* st) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->count = 0;
+    st->done = 0;
+  }
 }
 
 // GPU_fill (src/kernels.cu:482-488).
@@ -519,6 +547,34 @@ extern "C" int psx_normalise(float* p, float mean, float sigma, uint64_t n, psx_
   PSX_CHECK_LAUNCH("k_normalise");
   return psx::ok();
 }
+
+namespace psx {
+int sum_sq_async(const float* x, uint64_t first, uint64_t n, void* work, double2* out, hipStream_t s) {
+  double2* partial = (double2*)work;
+  const unsigned g = psx::grid_for(n - first, kBlock, kReduceBlocks);
+  k_sum_sq<<<g, kBlock, 0, s>>>(x, first, n, partial);
+  PSX_CHECK_LAUNCH("k_sum_sq");
+  k_sum_final<<<1, kBlock, 0, s>>>(partial, g, out);
+  PSX_CHECK_LAUNCH("k_sum_final");
+  return PSX_OK;
+}
+
+int remove_baseline_async(float* x, uint64_t n, void* work, BaselineState* st, int steps, hipStream_t s) {
+  if (!x || !work || !st || n == 0 || steps < 1) return psx::fail(PSX_ERR_ARG, "remove_baseline_async: bad arguments");
+  double2* sums = (double2*)work + kReduceBlocks;
+  k_baseline_init<<<1, 64, 0, s>>>(st);
+  PSX_CHECK_LAUNCH("k_baseline_init");
+  for (int i = 0; i < steps; ++i) {
+    int rc = sum_sq_async(x, 0, n, work, sums, s);
+    if (rc) return rc;
+    k_subtract_dev<<<psx::grid_for(n, kBlock), kBlock, 0, s>>>(x, n, sums, st);
+    PSX_CHECK_LAUNCH("k_subtract_dev");
+    k_baseline_step<<<1, 64, 0, s>>>(n, sums, st);
+    PSX_CHECK_LAUNCH("k_baseline_step");
+  }
+  return PSX_OK;
+}
+}  // namespace psx
 
 // GPU_remove_baseline (src/kernels.cu:463-479).
 extern "C" int psx_remove_baseline(float* x, uint64_t n, void* work, psx_stream s) {

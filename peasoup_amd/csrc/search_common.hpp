@@ -7,6 +7,7 @@
 #include "peasoup/candidates.hpp"
 
 namespace psx {
+
 namespace search {
 
 // Candidate lists cross the C ABI as a "forest": each top-level candidate is

@@ -228,6 +228,29 @@ def test_row_layout_unavailable_is_an_error(psx):
         W.batch_search(w.search_args(), series, w.dm_list(), None, opts)
 
 
+def test_async_prologue_baseline_loop_matches_worker(psx):
+    """The batched search runs the DM prologues ahead on their own stream
+    with a fixed number of device iterations of remove_baseline's loop
+    (kernels.cu:463-479); a DM whose loop has not stopped by then is redone
+    with the host-driven loop.  Series needing 1, 2, 3 and 102 iterations
+    (the loop's cap: a quarter of the samples at 1e10) give the plain
+    Worker's candidates exactly."""
+    import peasoup_amd.workloads as W
+    n = 1 << 17
+    w = W.Workload("bl", 4, n, 64e-6, 0x5EA5D0, 0.0, -20.0, 20.0,
+                   injections=[W.Injection(d, 0.0111, 25.0, 15.0, 0.2) for d in range(4)])
+    series = W.generate_host(w).astype(np.float32)
+    series[0] -= np.float32(series[0].mean())                    # ~1 iteration
+    series[1] += np.float32(5.0)                                 # 2 iterations
+    series[2] = np.float32(2.0 ** 22) + np.float32(0.5) * (np.arange(n) % 2).astype(np.float32)  # 3
+    series[3][: n // 4] = np.float32(1e10)                       # loop cap: host fallback
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs, forest=True)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(8), forest=True)
+    assert len(ref) > 0
+    assert same(got, ref)
+
+
 def test_batched_overflow_is_an_error(psx):
     """A threshold below the noise floor floods the candidate buffer: the
     search must fail loudly rather than drop candidates."""
