@@ -68,11 +68,12 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2 (&tw)
         if constexpr (TWS == TW_PREFETCH)
           w = tw[b * (R - 1) + r - 1];
         else if constexpr (TWS == TW_LDS)
-          w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[k * r * TS];
+          w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[tw_slot(k * r * TS)];
         else if constexpr (TWS == TW_LDS_HALF) {
           const int m = k * r * TS;
           const bool upper = m >= L / 2;
-          w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[upper ? m - L / 2 : m];
+          w = reinterpret_cast<const float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4)[tw_slot(upper ? m - L / 2
+                                                                                                         : m)];
           if (upper) w = make_float2(-w.x, -w.y);
         }
         else
@@ -210,7 +211,7 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
     // the first use follows the stage-0 exchange and its barriers
     constexpr int NTW = twiddle_source(L, C, MODE) == TW_LDS ? L : L / 2;
     float2* ltw = reinterpret_cast<float2*>(lds + lds_bytes(L, C, split_planes(MODE)) / 4);
-    for (int i = t; i < NTW; i += NT) ltw[i] = ldg2(a.twL, (uint32_t)i);
+    for (int i = t; i < NTW; i += NT) ltw[tw_slot(i)] = ldg2(a.twL, (uint32_t)i);
   }
   if constexpr (RS) {
     // resampleII fused into the first read (src/kernels.cu:342-353); the

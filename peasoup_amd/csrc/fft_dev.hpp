@@ -80,18 +80,28 @@ __host__ __device__ constexpr int waves_per_simd(int L, int C, int mode) {
 //                hides behind it);
 //   TW_GLOBAL    loads at the point of use.
 enum TwiddleSource { TW_GLOBAL = 0, TW_LDS = 1, TW_PREFETCH = 2, TW_LDS_HALF = 3 };
+// LDS twiddle tables keep entry m at slot m + m/32: a stage reads W[k*r*TS]
+// for consecutive k, i.e. strides of r*TS entries, which without the pad put
+// a wave's lanes on a few banks (all of them on one bank for 16-point
+// spans); with it the reads average under the 4 cycles of a conflict-free
+// 64-bit access in a bank model of the access patterns.
+__host__ __device__ constexpr int tw_slot(int m) { return m + (m >> 5); }
+__host__ __device__ constexpr int tw_table_bytes(int entries) { return 8 * tw_slot(entries); }
 __host__ __device__ constexpr int twiddle_source(int L, int C, int mode) {
   const int base = lds_bytes(L, C, split_planes(mode));
   if (num_stages(L) < 2) return TW_GLOBAL;
-  if (base + 8 * L <= 163840 && wgs_per_cu(L, C, base + 8 * L) == wgs_per_cu(L, C, base)) return TW_LDS;
-  if (base + 4 * L <= 163840 && wgs_per_cu(L, C, base + 4 * L) == wgs_per_cu(L, C, base)) return TW_LDS_HALF;
+  const int full = tw_table_bytes(L), half = tw_table_bytes(L / 2);
+  if (base + full <= 163840 && wgs_per_cu(L, C, base + full) == wgs_per_cu(L, C, base)) return TW_LDS;
+  if (base + half <= 163840 && wgs_per_cu(L, C, base + half) == wgs_per_cu(L, C, base)) return TW_LDS_HALF;
   if (points_per_thread(L) <= 16 && 512 / waves_per_simd(L, C, mode) >= 128) return TW_PREFETCH;
   return TW_GLOBAL;
 }
 // Dynamic LDS of a pass kernel: the exchange buffer plus the twiddle table.
 __host__ __device__ constexpr int pass_lds_bytes(int L, int C, int mode) {
   return lds_bytes(L, C, split_planes(mode)) +
-         (twiddle_source(L, C, mode) == TW_LDS ? 8 * L : (twiddle_source(L, C, mode) == TW_LDS_HALF ? 4 * L : 0));
+         (twiddle_source(L, C, mode) == TW_LDS
+              ? tw_table_bytes(L)
+              : (twiddle_source(L, C, mode) == TW_LDS_HALF ? tw_table_bytes(L / 2) : 0));
 }
 
 // ------------------------------------------------- global memory access
