@@ -129,6 +129,8 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
                                                   const TrialDesc* __restrict__ descs) {
   __shared__ float lo_t[kColTile][kRowTile + 1];
   __shared__ float up_t[kColTile][kRowTile + 1];
+  __shared__ float2 edge_lo[kB / 64][kColTile];  // X of each wave's last lower row
+  __shared__ float2 edge_hi[kB / 64][kColTile];  // X of each wave's first upper row
   const uint32_t tr = blockIdx.z;
   const uint32_t M = R * L, half_rows = R / 2;
   const float2* y = Y + (uint64_t)tr * M;
@@ -137,28 +139,48 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   const uint32_t ra = 1 + blockIdx.y * kRowTile, ea = blockIdx.x * kColTile;
   const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint32_t e = ea + lane, emir = L - 1 - e;
-  const uint32_t r0 = ra + wv * (kRowTile / 4);
-  // Branch-free sweep (so the loads of later rows issue early): rows past
-  // R/2 are never used and load row R/2 instead; results of rows outside
-  // [1, R/2) go to the spare slot kRowTile of each LDS column.
-  float2 xlo_prev = make_float2(0.f, 0.f), xhi_prev = make_float2(0.f, 0.f);
-#pragma unroll
-  for (int i = 0; i < kRowTile / 4 + 2; ++i) {
-    const uint32_t q = min(r0 - 1 + (uint32_t)i, half_rows);
+  constexpr int kWaveRows = kRowTile / (kB / 64);
+  const uint32_t r0 = ra + wv * kWaveRows;
+  // pair row q -> (X(q, e), X(R-q, L-1-e)); rows past R/2 are never used
+  // and load row R/2 instead (in bounds, results discarded)
+  auto pair_row = [&](uint32_t qq, float2& xlo, float2& xhi) {
+    const uint32_t q = min(qq, half_rows);
     const uint32_t ih = q == 0 ? (e == 0 ? 0 : L - e) : (R - q) * L + emir;
-    const float2 zl = y[q * L + e];
-    const float2 zh = y[ih];
+    untangle_pair(y[q * L + e], y[ih], q + R * e, M, rt, xlo, xhi);
+  };
+  // results of rows outside [1, R/2) go to the spare slot kRowTile
+  auto slot = [&](uint32_t qq) { return qq < half_rows ? qq - ra : (uint32_t)kRowTile; };
+  // Branch-free sweep over the wave's own rows (so the loads of later rows
+  // issue early); the interbin neighbours across wave boundaries come from
+  // the adjacent waves through LDS, and only the workgroup's outer halo
+  // rows (ra-1 below, ra+64 above) are computed twice.
+  float2 xlo_prev = make_float2(0.f, 0.f), xhi_prev = make_float2(0.f, 0.f);
+  float2 xlo_first = xlo_prev, xhi_last = xhi_prev;
+#pragma unroll
+  for (int i = 0; i < kWaveRows; ++i) {
+    const uint32_t qq = r0 + (uint32_t)i;
     float2 xlo, xhi;
-    untangle_pair(zl, zh, q + R * e, M, rt, xlo, xhi);
-    const uint32_t qq = r0 - 1 + (uint32_t)i;
-    // lower bin (q, e) for q in [r0, r0+16)
-    if (i >= 1 && i <= kRowTile / 4)
-      lo_t[lane][qq < half_rows ? qq - ra : kRowTile] = normalised(xlo, xlo_prev, mean_n, std_n);
-    // upper bin (R-q', L-1-e) for q' = q-1 in [r0, r0+16)
-    if (i >= 2) up_t[lane][qq - 1 < half_rows ? qq - 1 - ra : kRowTile] = normalised(xhi_prev, xhi, mean_n, std_n);
+    pair_row(qq, xlo, xhi);
+    if (i == 0) {
+      xlo_first = xlo;
+      edge_hi[wv][lane] = xhi;
+    } else {
+      lo_t[lane][slot(qq)] = normalised(xlo, xlo_prev, mean_n, std_n);       // P(q)
+      up_t[lane][slot(qq - 1)] = normalised(xhi_prev, xhi, mean_n, std_n);   // P(R-(q-1))
+    }
     xlo_prev = xlo;
     xhi_prev = xhi;
   }
+  edge_lo[wv][lane] = xlo_prev;
+  xhi_last = xhi_prev;
+  float2 below = make_float2(0.f, 0.f), above = make_float2(0.f, 0.f), unused;
+  if (wv == 0) pair_row(ra - 1, below, unused);
+  if (wv == kB / 64 - 1) pair_row(ra + kRowTile, unused, above);
+  __syncthreads();
+  if (wv > 0) below = edge_lo[wv - 1][lane];
+  if (wv < kB / 64 - 1) above = edge_hi[wv + 1][lane];
+  lo_t[lane][slot(r0)] = normalised(xlo_first, below, mean_n, std_n);
+  up_t[lane][slot(r0 + kWaveRows - 1)] = normalised(xhi_last, above, mean_n, std_n);
   __syncthreads();
   const uint32_t rr = threadIdx.x & 63u;
   const uint32_t r = ra + rr;
