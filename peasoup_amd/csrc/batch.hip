@@ -112,6 +112,8 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
 // Rows 0 and R/2 and bin M pair within their own row: k_spec_edge.
 constexpr int kRowTile = 64;  // lower rows per workgroup (16 per wave)
 constexpr int kColTile = 64;  // columns per workgroup (one per lane)
+// the write-out gives each lane one row of the tile
+static_assert(kRowTile == 64 && kColTile == 64, "k_spec_rows writes one tile row per lane");
 
 // X[k] of the pair (Z[k], Z[M-k]) exactly as untangled_bin computes it:
 // the root of min(k, M-k), mirrored for the upper bin.
