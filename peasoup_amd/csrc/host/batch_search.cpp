@@ -236,8 +236,7 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
         // with the host-driven prologue once ps is idle (its buffers are
         // shared), from its original input
         hip_check(hipStreamSynchronize(ps), "prologue stream");
-        eng->prologue(input_of((uint32_t)d, s), nreal, const_cast<float*>(tims[d]), &means[d], &stds[d],
-                      zap.get());
+        eng->prologue(input_of((uint32_t)d, s), nreal, d_tims + d * N, &means[d], &stds[d], zap.get());
         hip_check(hipStreamSynchronize(s), "prologue");
       } else {
         eng->stats_of(h_res[d], &means[d], &stds[d]);

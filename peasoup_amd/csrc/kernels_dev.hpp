@@ -31,12 +31,12 @@ __device__ inline uint64_t resample_index(uint64_t i, uint64_t n, double af) {
 
 // The same index for series shorter than 2^31 samples, in 32-bit integers
 // (identical values: every operand is exactly representable either way).
+// n >= 1; the n edge clamps with one unsigned min.
 __device__ inline uint32_t resample_index32(uint32_t i, uint32_t n, double dn, double af) {
   const double di = (double)i;
   double v = fma(di * af, di - dn, di);
   v = v > 0.0 ? v : 0.0;
-  const uint32_t j = (uint32_t)rint(v);
-  return j < n ? j : n - 1;
+  return min((uint32_t)rint(v), n - 1);
 }
 
 // bin_interbin_series_kernel (src/kernels.cu:250-257): the 0.5 is a double

@@ -10,11 +10,18 @@ Two inputs:
   1 GiB each way): factor = counter reading of the copy / 1 GiB, and a
   kernel's traffic = its reading / factor.  The per-trial traffic of the
   batched stages is written to profiles/traffic.json, which bench.py reports
-  as roofline.traffic.
+  as roofline.traffic;
+* one counter run of ``bench.py --profile-run`` with the SQ instruction
+  counters (``--pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_VMEM_RD
+  GRBM_GUI_ACTIVE``) -> instructions per wave of each fused kernel and the
+  fraction of the CUs' issue cycles its VALU and LDS instructions take (a
+  CU issues at most one wave64 VALU instruction per cycle over its four
+  SIMDs; GRBM_GUI_ACTIVE counts the busy cycles of every XCD).
 
 Usage:
   python -m peasoup_amd.profiling stats <dir>            # kernel stats table
   python -m peasoup_amd.profiling traffic <fetch_dir> <write_dir> --n 8388608 --trials 16 [--out traffic.json]
+  python -m peasoup_amd.profiling issue <dir> [--cus 256 --xcds 8]
 """
 import argparse
 import csv
@@ -149,6 +156,48 @@ def traffic(fetch_dir, write_dir, trials):
     return out
 
 
+def _dispatches(d):
+    """{dispatch_id: (kernel, grid, {counter: value})} of a counter run."""
+    res = {}
+    for r in _rows(os.path.join(d, "**", "*counter_collection.csv")):
+        i = int(_col(r, "Dispatch_Id"))
+        e = res.setdefault(i, (short_name(_col(r, "Kernel_Name")), int(_col(r, "Grid_Size")), {}))
+        c = _col(r, "Counter_Name")
+        e[2][c] = e[2].get(c, 0.0) + float(_col(r, "Counter_Value"))
+    return res
+
+
+def issue(d, cus=256, xcds=8):
+    """Per batched stage: instructions per wave and the share of the CUs'
+    issue cycles taken by VALU and LDS instructions (averaged over the
+    stage's batched launches)."""
+    disp = _dispatches(d)
+    first = min((i for i, (k, _, _) in disp.items() if stage_of(k) == "fft_pass0_resample"), default=0)
+    groups = {}
+    for i, (k, g, c) in disp.items():
+        if i >= first and stage_of(k) is not None:
+            groups.setdefault(k, []).append((g, i, c))
+    per, fft_rank = {}, []
+    for k, vals in groups.items():
+        gmax = max(g for g, _, _ in vals)
+        big = [(i, c) for g, i, c in vals if g == gmax]
+        avg = {n: sum(c[n] for _, c in big) / len(big) for n in big[0][1]}
+        waves = avg["SQ_WAVES"]
+        cycles = avg["GRBM_GUI_ACTIVE"] / xcds
+        e = {"kernel": k, "launches": len(big), "waves": waves, "cycles": cycles,
+             "valu_per_wave": avg["SQ_INSTS_VALU"] / waves, "lds_per_wave": avg["SQ_INSTS_LDS"] / waves,
+             "vmem_rd_per_wave": avg["SQ_INSTS_VMEM_RD"] / waves,
+             "valu_issue": avg["SQ_INSTS_VALU"] / (cus * cycles), "lds_issue": avg["SQ_INSTS_LDS"] / (cus * cycles)}
+        st = stage_of(k)
+        if st == "fft_pass":
+            fft_rank.append((min(i for i, _ in big), e))
+        else:
+            per[st] = e
+    for q, (_, e) in enumerate(sorted(fft_rank, key=lambda x: x[0]), start=1):
+        per["fft_pass%d" % q] = e
+    return per
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -161,7 +210,19 @@ def main(argv=None):
     s2.add_argument("--n", type=int, required=True)
     s2.add_argument("--trials", type=int, required=True)
     s2.add_argument("--out")
+    s3 = sub.add_parser("issue")
+    s3.add_argument("dir")
+    s3.add_argument("--cus", type=int, default=256)
+    s3.add_argument("--xcds", type=int, default=8)
     a = ap.parse_args(argv)
+    if a.cmd == "issue":
+        print("%-20s %-36s %9s %10s %9s %8s %9s %10s %9s" % ("stage", "kernel", "waves", "cycles", "VALU/wave",
+                                                           "LDS/wave", "VMEM/wave", "VALU issue", "LDS issue"))
+        for st, e in sorted(issue(a.dir, a.cus, a.xcds).items()):
+            print("%-20s %-36s %9d %10.0f %9.0f %8.1f %9.1f %10.2f %9.2f" % (
+                st, e["kernel"][:36], e["waves"], e["cycles"], e["valu_per_wave"], e["lds_per_wave"],
+                e["vmem_rd_per_wave"], e["valu_issue"], e["lds_issue"]))
+        return
     if a.cmd == "stats":
         print("per kernel (rocprofv3 --stats):")
         print("%-60s %8s %12s %12s %7s" % ("kernel", "calls", "avg_us", "total_ms", "pct"))

@@ -55,3 +55,29 @@ def test_traffic_is_copy_calibrated_and_skips_the_prologue(tmp_path):
     assert st["fft_pass1"]["FETCH_SIZE"] == 5.0 * 1024 * 1024
     assert st["spec_post"]["bytes_per_launch"] == 2.0 * 1024 * 1024 + 1.0 * 1024 * 1024
     assert st["spec_post"]["bytes_per_trial"] == st["spec_post"]["bytes_per_launch"] / 4
+
+
+def test_issue_reports_instructions_per_wave_and_issue_share(tmp_path):
+    # one prologue FFT (before the batched launches) and one batched launch
+    # of each of two stages; every counter in one file
+    rows = []
+    for disp, grid, name, waves, valu, lds, vmem, gui in (
+            (1, 262144, "void k_fft_pass<1024, 16, 1, false>(psx::fft::PassArgs)", 10, 1, 1, 1, 8.0),
+            (2, 8388608, "void k_fft_pass<1024, 16, 1, true>(psx::fft::PassArgs)", 1000, 1200000, 100000, 60000,
+             8.0 * 10000),
+            (3, 4194304, "void psx::batch::k_spec_rows(float2 const*)", 500, 1000000, 30000, 30000, 8.0 * 5000)):
+        for c, v in (("SQ_WAVES", waves), ("SQ_INSTS_VALU", valu), ("SQ_INSTS_LDS", lds),
+                     ("SQ_INSTS_VMEM_RD", vmem), ("GRBM_GUI_ACTIVE", gui)):
+            rows.append((disp, grid, name, c, v))
+    path = tmp_path / "s" / "a" / "s_counter_collection.csv"
+    os.makedirs(os.path.dirname(str(path)))
+    with open(str(path), "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Dispatch_Id", "Grid_Size", "Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writerows(rows)
+    st = pr.issue(str(tmp_path / "s"), cus=100, xcds=8)
+    assert set(st) == {"fft_pass0_resample", "spec_post"}
+    p0 = st["fft_pass0_resample"]
+    assert p0["cycles"] == 10000 and p0["valu_per_wave"] == 1200 and p0["lds_per_wave"] == 100
+    assert abs(p0["valu_issue"] - 1200000 / (100 * 10000)) < 1e-12
+    assert abs(st["spec_post"]["lds_issue"] - 30000 / (100 * 5000)) < 1e-12
