@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--ndm", type=int, default=64)
     ap.add_argument("--batch", type=int, default=0, help="trials per fused batch (0 = tuned/default)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-keep", action="store_true",
+                    help="build and free the search engine in every step instead of keeping it between steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -243,7 +245,7 @@ def main():
     ntrials = w.ndm * accs.size
     tuned = tuning.lookup(w.size, dev["name"])
     batch = a.batch or tuned.get("batch", 16)
-    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"))
+    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"), keep_engine=not a.no_keep)
     log("workload %s: %d DMs x %d accels = %d trials, batch %d, decomp %s" %
         (w.name, w.ndm, accs.size, ntrials, batch, tuned.get("decomp")))
 
@@ -286,6 +288,7 @@ def main():
     dt = time.perf_counter() - t0
     value = a.steps * ntrials / dt
     log("timed %d steps: %.3f s -> %.1f trials/s" % (a.steps, dt, value))
+    W.release_batch_engine()
 
     # per-launch profile of one batch (HIP events on the batch stream)
     stage_ms, desc = profile_stages(P, W, args_c, opts, d_series.ptr, w.size, accs[:batch], reps=10)
@@ -311,7 +314,8 @@ def main():
                                "accel plan -100..+100 m/s^2 tol 1.1 (127 trials/DM); 4 harmonic folds; S/N 7"
                                % (w.ndm, len(w.injections)),
                    "fft_size": w.size, "ndm": w.ndm, "trials_per_step": ntrials, "batch": batch,
-                   "fft_plan": desc, "graphs": not a.no_graph, "device": dev["name"]},
+                   "fft_plan": desc, "graphs": not a.no_graph, "engine_kept_between_steps": not a.no_keep,
+                   "device": dev["name"]},
         "roofline": roofline, "stages": stages,
         "algorithmic_GBps": round(value * (12 * w.size + 68 * (w.size // 2 + 1)) / 1e9, 1),
         "candidates": int(first.size) if first is not None else None,

@@ -245,6 +245,8 @@ typedef struct psx_batch_opts {
   int32_t fft_len[3];      /* pass lengths (product = n/2) */
   int32_t fft_cols[3];     /* column tile per pass */
   int32_t spectrum_layout; /* PSX_SPECTRUM_*: how the last FFT pass hands over the spectra */
+  int32_t keep_engine;     /* keep the engine for the next call with the same arguments and
+                              options (psx_batch_release frees it) */
 } psx_batch_opts;
 
 /* Spectrum hand-over of the batched path.  AUTO: a two-pass plan whose last
@@ -263,6 +265,14 @@ enum { PSX_SPECTRUM_AUTO = 0, PSX_SPECTRUM_NATURAL = 1, PSX_SPECTRUM_ROWS = 2 };
 int psx_batch_search(const psx_search_args* args, const psx_batch_opts* opts, const float* series,
                      int series_on_device, uint64_t nsamps, const float* dm_list, uint32_t ndm, const float* accs,
                      uint32_t naccs, psx_candidate* out, uint32_t cap, uint32_t* count, double* seconds);
+
+/* Frees the engine a psx_batch_search call with keep_engine left behind
+ * (its device buffers, FFT plans and launch graphs; one engine is kept per
+ * process, replaced by the next keeping call with other arguments).  No
+ * counterpart in the reference, whose Worker builds its transforms per DM
+ * (pipeline_multi.cu:111-145); for pipelines that search many segments of
+ * one shape. */
+int psx_batch_release(void);
 
 /* Times each launch of one fused batch (min(naccs, batch) trials over the
  * first series of d_series, after its prologue) with HIP events on the batch

@@ -194,7 +194,14 @@ void Engine::stats_of(const PrologueResult& res, float* mean, float* std) const 
   *std = std::sqrt(r * r - m * m);
 }
 
-void Engine::enqueue(Slot& sl, uint32_t n) {
+void Engine::enqueue(Slot& sl, uint32_t n) { launch(sl, n, true); }
+
+void Engine::prepare(uint32_t n) {
+  if (!use_graph || n == 0 || n > B) return;
+  for (Slot& sl : slots) launch(sl, n, false);
+}
+
+void Engine::launch(Slot& sl, uint32_t n, bool run) {
   auto record = [&]() {
     hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s),
            "descriptor upload");
@@ -223,6 +230,7 @@ void Engine::enqueue(Slot& sl, uint32_t n) {
       (void)hipGraphDestroy(g);
       it = sl.graphs.emplace(n, ge).first;
     }
+    if (!run) return;
     hip_ok(hipGraphLaunch(it->second, s), "graph launch");
   } else {
     record();

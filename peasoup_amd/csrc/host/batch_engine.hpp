@@ -68,6 +68,10 @@ class Engine {
   // collects the oldest first when both slots are busy (its results are
   // queued for the next collect()).
   void submit(const TrialWork* trials, uint32_t n);
+  // Captures and instantiates both slots' launch graphs for batches of n
+  // trials now, so that the first submits do not instantiate while other
+  // streams' work is queued (instantiation waits for the device).
+  void prepare(uint32_t n);
   // Waits for the oldest in-flight batch and returns its per-trial
   // harmonic-distilled candidate lists (trial order of the submit).
   bool collect(std::vector<std::vector<Candidate>>& per_trial);
@@ -112,6 +116,8 @@ class Engine {
     std::map<uint32_t, hipGraphExec_t> graphs;
   };
   void enqueue(Slot& sl, uint32_t n);
+  // enqueue's body; run = false only builds the slot's graph for n
+  void launch(Slot& sl, uint32_t n, bool run);
   void spectrum(Slot& sl, uint32_t n);
   // the fused FFT leaves the transposed row layout (rows_R rows of rows_L)
   bool rows_out = false;

@@ -10,6 +10,7 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -113,7 +114,91 @@ bool engine_options(const psx_batch_opts* opts, EngineOptions& eo, psx::fft::Dec
   }
   return true;
 }
+
+// The engine (with its stream) that the last call asking for it left
+// behind: its device buffers, FFT plans and launch graphs serve the next
+// call with the same device, search arguments and options.  A call takes it
+// out of the cache while it runs, so concurrent calls never share one.
+struct EngineKey {
+  int device = -1;
+  psx_search_args args;
+  uint32_t batch = 0, cand_capacity = 0;
+  bool use_graph = false, forced = false;
+  int spectrum_layout = 0;
+  psx::fft::Decomp decomp;
+};
+EngineKey engine_key(const psx_search_args& a, const EngineOptions& eo) {
+  EngineKey k;
+  std::memset(&k.args, 0, sizeof(k.args));
+  k.args = a;
+  (void)hipGetDevice(&k.device);
+  k.batch = eo.batch;
+  k.cand_capacity = eo.cand_capacity;
+  k.use_graph = eo.use_graph;
+  k.spectrum_layout = eo.spectrum_layout;
+  k.forced = eo.decomp != nullptr;
+  if (eo.decomp) k.decomp = *eo.decomp;
+  return k;
+}
+bool same_key(const EngineKey& x, const EngineKey& y) {
+  return x.device == y.device && std::memcmp(&x.args, &y.args, sizeof(x.args)) == 0 && x.batch == y.batch &&
+         x.cand_capacity == y.cand_capacity && x.use_graph == y.use_graph && x.forced == y.forced &&
+         x.spectrum_layout == y.spectrum_layout &&
+         (!x.forced || (x.decomp.npass == y.decomp.npass &&
+                        std::memcmp(x.decomp.L, y.decomp.L, sizeof(x.decomp.L)) == 0 &&
+                        std::memcmp(x.decomp.C, y.decomp.C, sizeof(x.decomp.C)) == 0));
+}
+struct KeptEngine {
+  std::unique_ptr<Engine> eng;
+  hipStream_t s = nullptr;
+  EngineKey key;
+  void reset() {
+    eng.reset();
+    if (s) (void)hipStreamDestroy(s);
+    s = nullptr;
+  }
+};
+// never destroyed: an engine still kept at exit is left to the process
+// teardown (freeing it from a static destructor could run after the HIP
+// runtime's own teardown)
+std::mutex& g_kept_mu = *new std::mutex;
+KeptEngine& g_kept = *new KeptEngine;
+
+// Takes the kept engine when its key matches (else leaves the cache alone).
+bool take_kept(const EngineKey& k, std::unique_ptr<Engine>& eng, hipStream_t& s) {
+  std::lock_guard<std::mutex> lock(g_kept_mu);
+  if (!g_kept.eng || !same_key(g_kept.key, k)) return false;
+  eng = std::move(g_kept.eng);
+  s = g_kept.s;
+  g_kept.s = nullptr;
+  return true;
+}
+// Keeps an idle engine; the one it replaces is freed outside the lock.
+void keep(std::unique_ptr<Engine> eng, hipStream_t s, const EngineKey& k) {
+  KeptEngine old;
+  {
+    std::lock_guard<std::mutex> lock(g_kept_mu);
+    old.eng = std::move(g_kept.eng);
+    old.s = g_kept.s;
+    g_kept.eng = std::move(eng);
+    g_kept.s = s;
+    g_kept.key = k;
+  }
+  old.reset();
+}
 }  // namespace
+
+extern "C" int psx_batch_release(void) {
+  KeptEngine old;
+  {
+    std::lock_guard<std::mutex> lock(g_kept_mu);
+    old.eng = std::move(g_kept.eng);
+    old.s = g_kept.s;
+    g_kept.s = nullptr;
+  }
+  old.reset();
+  return psx::ok();
+}
 
 extern "C" int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
                                  uint64_t nsamps, const float* accs, uint32_t naccs, int reps, double* stage_ms,
@@ -184,10 +269,15 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
     EngineOptions eo;
     psx::fft::Decomp forced;
     engine_options(opts, eo, forced);
-    PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    PSX_CHECK_HIP(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+    const bool keep_engine = opts && opts->keep_engine;
+    const EngineKey key = engine_key(*args, eo);
     const uint64_t N = args->size;
-    std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
+    std::unique_ptr<Engine> eng;
+    if (!keep_engine || !take_kept(key, eng, s)) {
+      PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      eng.reset(new Engine(*args, eo, s));
+    }
+    PSX_CHECK_HIP(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
     // declared after the engine, so destroyed first on every exit path: no
     // prologue work may still be queued on ps when the engine frees the
     // buffers it uses (a DM without trials is never waited for)
@@ -218,16 +308,24 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
       hip_check(hipMemcpyAsync(d_stage, src, sizeof(float) * nreal, hipMemcpyHostToDevice, st), "series upload");
       return (const float*)d_stage;
     };
+    uint64_t ntrials = 0;
+    for (uint32_t d = 0; d < ndm; ++d) {
+      if (accs)
+        acc_lists[d].assign(accs, accs + naccs);
+      else
+        plan.generate_accel_list(dms[d], args->cdm, acc_lists[d]);
+      ntrials += acc_lists[d].size();
+    }
+    // the launch graphs of the full and the last batch exist before any
+    // prologue is queued: building one later waits for the queued prologues
+    eng->prepare((uint32_t)std::min<uint64_t>(eng->batch(), ntrials));
+    if (ntrials > eng->batch()) eng->prepare((uint32_t)(ntrials % eng->batch()));
     for (uint32_t d = 0; d < ndm; ++d) {
       float* tim = d_tims + (uint64_t)d * N;
       eng->prologue_async(input_of(d, ps), nreal, tim, &h_res[d], ps, zap.get());
       PSX_CHECK_HIP(hipEventRecord(done_ev[d], ps));
       tims[d] = tim;
       idx[d] = (int)d;
-      if (accs)
-        acc_lists[d].assign(accs, accs + naccs);
-      else
-        plan.generate_accel_list(dms[d], args->cdm, acc_lists[d]);
     }
     auto ready = [&](size_t d) {
       hip_check(hipEventSynchronize(done_ev[d]), "prologue event");
@@ -255,7 +353,12 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
       seconds[2] = secs(t0, t2);
     }
     hip_check(hipStreamSynchronize(ps), "prologue stream");
-    eng.reset();
+    if (keep_engine) {
+      keep(std::move(eng), s, key);
+      s = nullptr;
+    } else {
+      eng.reset();
+    }
     release();
     return psx::search::flatten(flat, out, cap, count) == PSX_OK ? psx::ok() : PSX_ERR_OVERFLOW;
   } catch (const std::exception& e) {

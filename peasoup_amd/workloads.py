@@ -170,15 +170,17 @@ def generate_host(w: Workload, trials=None):
 class BatchOpts(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("cand_capacity", C.c_uint32), ("use_graph", C.c_int32),
                 ("fft_npass", C.c_int32), ("fft_len", C.c_int32 * 3), ("fft_cols", C.c_int32 * 3),
-                ("spectrum_layout", C.c_int32)]
+                ("spectrum_layout", C.c_int32), ("keep_engine", C.c_int32)]
 
 
 SPECTRUM_AUTO, SPECTRUM_NATURAL, SPECTRUM_ROWS = 0, 1, 2
 
 
-def make_batch_opts(batch=16, cand_capacity=0, use_graph=True, decomp=None, spectrum_layout=SPECTRUM_AUTO):
+def make_batch_opts(batch=16, cand_capacity=0, use_graph=True, decomp=None, spectrum_layout=SPECTRUM_AUTO,
+                    keep_engine=False):
     o = BatchOpts(batch, cand_capacity, 1 if use_graph else 0, 0)
     o.spectrum_layout = spectrum_layout
+    o.keep_engine = 1 if keep_engine else 0
     if decomp:
         o.fft_npass = len(decomp)
         for q, (L, Cc) in enumerate(decomp):
@@ -188,6 +190,7 @@ def make_batch_opts(batch=16, cand_capacity=0, use_graph=True, decomp=None, spec
 
 
 P.register_signatures({
+    "psx_batch_release": (C.c_int, []),
     "psx_batch_search": (C.c_int, [C.POINTER(SearchArgs), C.POINTER(BatchOpts), C.c_void_p, C.c_int, C.c_uint64,
                                    C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                    C.POINTER(C.c_uint32), C.c_void_p]),
@@ -224,6 +227,11 @@ def worker_search(args, series, dm_list, accs=None, cap=1 << 20, forest=False):
                                       secs.ctypes.data))
     out = out[:cnt.value]
     return (out if forest else top_level(out)), secs
+
+
+def release_batch_engine():
+    """Frees the engine that batch_search calls with keep_engine left behind."""
+    P.check(P.lib().psx_batch_release())
 
 
 def batch_search(args, series, dm_list, accs=None, opts=None, nsamps=None, cap=1 << 20, forest=False):

@@ -32,6 +32,35 @@ def test_batched_equals_worker(psx, cfg1, batch, graph):
     assert same(got, ref)
 
 
+def test_kept_engine_gives_the_same_candidates(psx, cfg1):
+    """keep_engine: a call leaves its engine behind, the next call with the
+    same arguments and options reuses it (buffers, plans, launch graphs),
+    calls with other options or another length replace it, and
+    psx_batch_release frees it; every call returns the Worker's candidates."""
+    import peasoup_amd.workloads as W
+    w, series, accs = cfg1
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs)
+    w2 = W.Workload("kept2", 3, 1 << 16, 64e-6, 0x5EA5E2, 5.0, -40.0, 40.0,
+                    injections=[W.Injection(1, 3.3e-3, 10.0, 15.0, 0.2)])
+    s2 = W.generate_host(w2)
+    a2 = W.accel_list(w2.size, w2.tsamp, w2.acc_lo, w2.acc_hi)
+    ref2, _ = W.worker_search(w2.search_args(), s2, w2.dm_list(), a2)
+    try:
+        for batch in (16, 16, 5, 5, 16):
+            got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs,
+                                    W.make_batch_opts(batch, keep_engine=True))
+            assert same(got, ref), batch
+        got2, _ = W.batch_search(w2.search_args(), s2, w2.dm_list(), a2, W.make_batch_opts(16, keep_engine=True))
+        assert same(got2, ref2)
+        got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(16, keep_engine=True))
+        assert same(got, ref)
+    finally:
+        W.release_batch_engine()
+    W.release_batch_engine()
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(16))
+    assert same(got, ref)
+
+
 def test_batched_vs_oracle(psx, orc, cfg1):
     import peasoup_amd.workloads as W
     w, series, accs = cfg1
