@@ -120,7 +120,7 @@ static_assert(kRowTile == 64 && kColTile == 64, "k_spec_rows writes one tile row
 __device__ inline void untangle_pair(float2 zk, float2 zm, uint32_t k, uint32_t M, const RootTable& rt, float2& xk,
                                      float2& xm) {
   const bool upper = 2 * k > M;
-  const float2 w = root(rt, (uint64_t)(upper ? M - k : k));
+  const float2 w = root32(rt, upper ? M - k : k);
   const float2 wm = make_float2(-w.x, w.y);
   xk = untangle_one(zk, zm, upper ? wm : w);
   xm = untangle_one(zm, zk, upper ? w : wm);
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   auto pair_row = [&](uint32_t qq, float2& xlo, float2& xhi) {
     const uint32_t q = min(qq, half_rows);
     const uint32_t ih = q == 0 ? (e == 0 ? 0 : L - e) : (R - q) * L + emir;
-    untangle_pair(y[q * L + e], y[ih], q + R * e, M, rt, xlo, xhi);
+    untangle_pair(ldg2(y, q * L + e), ldg2(y, ih), q + R * e, M, rt, xlo, xhi);
   };
   // results of rows outside [1, R/2) go to the spare slot kRowTile
   auto slot = [&](uint32_t qq) { return qq < half_rows ? qq - ra : (uint32_t)kRowTile; };
@@ -189,8 +189,8 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   if (r < half_rows) {
 #pragma unroll 4
     for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
-      p[r + R * (ea + j)] = lo_t[j][rr];
-      p[(R - r) + R * (L - 1 - (ea + j))] = up_t[j][rr];
+      stg1(p, r + R * (ea + j), lo_t[j][rr]);
+      stg1(p, (R - r) + R * (L - 1 - (ea + j)), up_t[j][rr]);
     }
   }
 }

@@ -122,6 +122,9 @@ __device__ inline float2 ldg2(const float2* base, uint32_t i) {
 __device__ inline float ldg1(const float* base, uint32_t i) {
   return *(const PSX_GLOBAL float*)((const PSX_GLOBAL char*)base + (uint64_t)(i * 4u));
 }
+__device__ inline void stg1(float* base, uint32_t i, float v) {
+  *(PSX_GLOBAL float*)((PSX_GLOBAL char*)base + (uint64_t)(i * 4u)) = v;
+}
 __device__ inline void stg2(float2* base, uint32_t i, float2 v) {
   psx_f2v w;
   w.x = v.x;

@@ -32,6 +32,10 @@ struct RootTable {
 };
 
 __device__ inline float2 root(const RootTable& t, uint64_t k) { return cmul(t.hi[k >> t.shift], t.lo[k & t.mask]); }
+// The same root for k < 2^29, loaded with 32-bit offsets from the tables.
+__device__ inline float2 root32(const RootTable& t, uint32_t k) {
+  return cmul(ldg2(t.hi, k >> t.shift), ldg2(t.lo, k & t.mask));
+}
 
 // R2C untangle of one bin: X = (zk + conj(zm))/2 - (i/2) w (zk - conj(zm)).
 __device__ inline float2 untangle_one(float2 zk, float2 zm, float2 w) {

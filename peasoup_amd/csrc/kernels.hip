@@ -165,8 +165,12 @@ __global__ void __launch_bounds__(kBlock) k_zap(float2* __restrict__ X, const fl
 // GPU_rms, src/kernels.cu:433-461, accumulate float terms in float).
 constexpr unsigned kReduceBlocks = 1024;
 
+// skip (optional): a stopped device baseline loop, whose later iterations
+// need no sums.
 __global__ void __launch_bounds__(kBlock) k_sum_sq(const float* __restrict__ x, uint64_t first, uint64_t n,
-                                                   double2* __restrict__ partial) {
+                                                   double2* __restrict__ partial,
+                                                   const psx::BaselineState* __restrict__ skip) {
+  if (skip && skip->done) return;
   double s1 = 0.0, s2 = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -190,7 +194,9 @@ __global__ void __launch_bounds__(kBlock) k_sum_sq(const float* __restrict__ x, 
 }
 
 __global__ void __launch_bounds__(kBlock) k_sum_final(const double2* __restrict__ partial, unsigned np,
-                                                      double2* __restrict__ out) {
+                                                      double2* __restrict__ out,
+                                                      const psx::BaselineState* __restrict__ skip) {
+  if (skip && skip->done) return;
   double s1 = 0.0, s2 = 0.0;
   for (unsigned i = threadIdx.x; i < np; i += blockDim.x) {
     s1 += partial[i].x;
@@ -514,9 +520,9 @@ static int sum_sq(const float* x, uint64_t first, uint64_t n, void* work, psx_st
   double2* partial = (double2*)work;
   double2* out = partial + kReduceBlocks;
   const unsigned g = psx::grid_for(n - first, kBlock, kReduceBlocks);
-  k_sum_sq<<<g, kBlock, 0, as_stream(s)>>>(x, first, n, partial);
+  k_sum_sq<<<g, kBlock, 0, as_stream(s)>>>(x, first, n, partial, nullptr);
   PSX_CHECK_LAUNCH("k_sum_sq");
-  k_sum_final<<<1, kBlock, 0, as_stream(s)>>>(partial, g, out);
+  k_sum_final<<<1, kBlock, 0, as_stream(s)>>>(partial, g, out, nullptr);
   PSX_CHECK_LAUNCH("k_sum_final");
   double2 h;
   PSX_CHECK_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, as_stream(s)));
@@ -549,12 +555,13 @@ extern "C" int psx_normalise(float* p, float mean, float sigma, uint64_t n, psx_
 }
 
 namespace psx {
-int sum_sq_async(const float* x, uint64_t first, uint64_t n, void* work, double2* out, hipStream_t s) {
+int sum_sq_async(const float* x, uint64_t first, uint64_t n, void* work, double2* out, hipStream_t s,
+                 const BaselineState* skip) {
   double2* partial = (double2*)work;
   const unsigned g = psx::grid_for(n - first, kBlock, kReduceBlocks);
-  k_sum_sq<<<g, kBlock, 0, s>>>(x, first, n, partial);
+  k_sum_sq<<<g, kBlock, 0, s>>>(x, first, n, partial, skip);
   PSX_CHECK_LAUNCH("k_sum_sq");
-  k_sum_final<<<1, kBlock, 0, s>>>(partial, g, out);
+  k_sum_final<<<1, kBlock, 0, s>>>(partial, g, out, skip);
   PSX_CHECK_LAUNCH("k_sum_final");
   return PSX_OK;
 }
@@ -565,7 +572,7 @@ int remove_baseline_async(float* x, uint64_t n, void* work, BaselineState* st, i
   k_baseline_init<<<1, 64, 0, s>>>(st);
   PSX_CHECK_LAUNCH("k_baseline_init");
   for (int i = 0; i < steps; ++i) {
-    int rc = sum_sq_async(x, 0, n, work, sums, s);
+    int rc = sum_sq_async(x, 0, n, work, sums, s, st);
     if (rc) return rc;
     k_subtract_dev<<<psx::grid_for(n, kBlock), kBlock, 0, s>>>(x, n, sums, st);
     PSX_CHECK_LAUNCH("k_subtract_dev");

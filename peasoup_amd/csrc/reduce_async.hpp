@@ -17,7 +17,10 @@ struct BaselineState {
   int32_t done;
 };
 int remove_baseline_async(float* x, uint64_t n, void* work, BaselineState* st, int steps, hipStream_t s);
-// Sums (x, x^2) over [first, n) into *out (device), as psx_mean_rms reduces.
-int sum_sq_async(const float* x, uint64_t first, uint64_t n, void* work, double2* out, hipStream_t s);
+// Sums (x, x^2) over [first, n) into *out (device), as psx_mean_rms reduces;
+// nothing is summed (and *out is left as it is) when skip points to a
+// stopped baseline loop.
+int sum_sq_async(const float* x, uint64_t first, uint64_t n, void* work, double2* out, hipStream_t s,
+                 const BaselineState* skip = nullptr);
 
 }  // namespace psx
