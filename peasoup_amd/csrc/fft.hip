@@ -808,6 +808,9 @@ bool rows_layout(const psx_fft_plan* p, uint32_t* rows, uint32_t* row_len) {
 }
 
 int num_passes(const psx_fft_plan* p) { return (int)p->passes.size(); }
+bool pass_in_place(const psx_fft_plan* p, int q) {
+  return q >= 0 && q < (int)p->passes.size() && p->passes[q].src == 1 && p->passes[q].dst == 1;
+}
 
 RootTable root_table(const psx_fft_plan* p) { return p->rt; }
 uint64_t complex_len(const psx_fft_plan* p) { return p->M; }

@@ -71,6 +71,9 @@ int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t n
 // first pass's length), instead of the natural Z[k].
 bool rows_layout(const psx_fft_plan* p, uint32_t* rows, uint32_t* row_len);
 int num_passes(const psx_fft_plan* p);
+// True when pass q transforms the work buffer in place (the middle pass of
+// three-pass plans): launching it again transforms its output again.
+bool pass_in_place(const psx_fft_plan* p, int q);
 RootTable root_table(const psx_fft_plan* p);
 uint64_t complex_len(const psx_fft_plan* p);
 const Decomp& plan_decomp(const psx_fft_plan* p);

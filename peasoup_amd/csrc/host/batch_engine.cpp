@@ -364,8 +364,11 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
   // Each stage is timed over kRun back-to-back launches of itself between
   // two events, in pipeline order (every launch recomputes the same output
   // from the same input), so the event markers' launch gaps are spread over
-  // kRun launches instead of landing on every stage of a single batch.
+  // kRun launches instead of landing on every stage of a single batch.  A
+  // pass that works in place is launched once: a second launch would
+  // transform its own output, and every later stage would time garbage.
   constexpr int kRun = 4;
+  auto runs = [&](int k) { return k < np && psx::fft::pass_in_place(plan, k) ? 1 : kRun; };
   auto stage = [&](int k) {
     if (k < np)
       psx_ok(psx::fft::exec_fused_pass(plan, k, sl.d_desc, n, Z, s, rows_out), "fused pass");
@@ -378,7 +381,7 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
     psx_ok(launch_zero(sl.d_count, s), "zero");
     hip_ok(hipEventRecord(ev[0], s), "event");
     for (int k = 0; k < nstage; ++k) {
-      for (int i = 0; i < (r == 0 ? 1 : kRun); ++i) stage(k);
+      for (int i = 0; i < (r == 0 ? 1 : runs(k)); ++i) stage(k);
       hip_ok(hipEventRecord(ev[k + 1], s), "event");
     }
     hip_ok(hipEventSynchronize(ev[nstage]), "event sync");
@@ -386,7 +389,7 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
     for (int k = 0; k < nstage; ++k) {
       float ms = 0;
       hip_ok(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "elapsed");
-      acc[k] += ms / kRun;
+      acc[k] += ms / runs(k);
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);

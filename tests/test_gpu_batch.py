@@ -257,6 +257,41 @@ def test_row_layout_unavailable_is_an_error(psx):
         W.batch_search(w.search_args(), series, w.dm_list(), None, opts)
 
 
+def test_profile_times_three_pass_plans_on_valid_spectra(psx):
+    """psx_batch_profile launches each stage several times back to back; the
+    middle pass of a three-pass plan works in place, so it is launched once
+    (a second launch would transform its own output and every later stage
+    would be timed on garbage: the harmonic pass then finds bins above
+    threshold everywhere and runs many times slower).  Its spectrum and
+    harmonic stages take about as long as a two-pass plan's."""
+    import ctypes as C
+    import peasoup_amd.workloads as W
+    n = 1 << 19
+    w = W.Workload("prof3", 1, n, 64e-6, 0x5EA5D9, 5.0, -50.0, 50.0)
+    d = W.generate_device(w)
+    accs = np.ascontiguousarray(W.accel_list(n, w.tsamp, -50.0, 50.0)[:16], np.float32)
+    psx.register_signatures({"psx_batch_profile": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
+                                                             C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
+                                                             C.c_int, C.c_void_p, C.POINTER(C.c_int), C.c_char_p,
+                                                             C.c_size_t])})
+    args = w.search_args()
+
+    def stages(decomp):
+        ms = np.zeros(8)
+        ns = C.c_int(8)
+        desc = C.create_string_buffer(128)
+        opts = W.make_batch_opts(16, decomp=decomp)
+        psx.check(psx.lib().psx_batch_profile(C.byref(args), C.byref(opts), d.ptr, n, accs.ctypes.data, accs.size,
+                                              3, ms.ctypes.data, C.byref(ns), desc, 128))
+        return ms[:ns.value]
+
+    two = stages([(512, 16), (512, 16)])
+    three = stages([(64, 16), (64, 16), (64, 16)])
+    assert len(two) == 4 and len(three) == 5
+    assert three[-1] < 4 * two[-1] + 0.05, (two, three)   # harmonic sums + threshold
+    assert three[-2] < 4 * two[-2] + 0.05, (two, three)   # spectrum pass
+
+
 def test_async_prologue_baseline_loop_matches_worker(psx):
     """The batched search runs the DM prologues ahead on their own stream
     with a fixed number of device iterations of remove_baseline's loop
