@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(kB) k_spec_edge(const float2* __restrict__ Y, 
 // harmonic_sum_kernel (src/kernels.cu:34-100) + device_find_peaks
 // (src/kernels.cu:391-423) for the fundamental and every fold, each over
 // its own PeakFinder window (peakfinder.hpp:78-83).  Survivors are appended
-// to `out` with one atomic per wave; the host orders them by bin.
+// to `out` with one atomic per bin row of a wave (all folds together); the
+// host orders them by bin.
 //
 // A wave covers kHarmWaveBins consecutive bins; lane l holds the D bins
 // base + l + 64d.  Fold k's harmonic j of bin i is bin (j*i + 2^(k-1)) >> k,
@@ -329,20 +330,27 @@ __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, 
 #pragma unroll
         for (int k = 0; k < NF; ++k) s[k + 1] = f[k];
       }
+      // one atomic for the wave's hits of all folds at this d
+      bool hit[NF + 1];
+      uint64_t mask[NF + 1];
+      uint32_t total = 0;
 #pragma unroll
       for (int nh = 0; nh <= NF; ++nh) {
-        const bool hit = valid && (int32_t)i >= w.start[nh] && (int32_t)i < w.end[nh] && s[nh] > w.thr;
-        const uint64_t mask = __ballot(hit);
-        if (mask) {
-          const int leader = __ffsll((long long)mask) - 1;
-          uint32_t b = 0;
-          if ((int)lane == leader) b = atomicAdd(count, (uint32_t)__popcll(mask));
-          b = __shfl(b, leader);
-          if (hit) {
-            const uint32_t pos = b + psx::mbcnt64(mask);
-            if (pos < cap) out[pos] = Cand16{slot, (uint32_t)nh, (int32_t)i, s[nh]};
-          }
+        hit[nh] = valid && (int32_t)i >= w.start[nh] && (int32_t)i < w.end[nh] && s[nh] > w.thr;
+        mask[nh] = __ballot(hit[nh]);
+        total += (uint32_t)__popcll(mask[nh]);
+      }
+      if (total == 0) continue;
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(count, total);
+      b = __shfl(b, 0);
+#pragma unroll
+      for (int nh = 0; nh <= NF; ++nh) {
+        if (hit[nh]) {
+          const uint32_t pos = b + psx::mbcnt64(mask[nh]);
+          if (pos < cap) out[pos] = Cand16{slot, (uint32_t)nh, (int32_t)i, s[nh]};
         }
+        b += (uint32_t)__popcll(mask[nh]);
       }
     }
   }

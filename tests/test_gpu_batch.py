@@ -292,6 +292,22 @@ def test_profile_times_three_pass_plans_on_valid_spectra(psx):
     assert three[-2] < 4 * two[-2] + 0.05, (two, three)   # spectrum pass
 
 
+def test_many_bins_above_threshold_match_worker(psx):
+    """A threshold low enough that most bins of every fold cross it: the
+    harmonic pass's append path (taken by every wave) and the host's
+    clustering of tens of thousands of bins per spectrum give the plain
+    Worker's candidates exactly."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("low", 2, 1 << 17, 64e-6, 0x5EA5DA, 5.0, -20.0, 20.0, min_snr=1.0,
+                   injections=[W.Injection(0, 0.0123, 5.0, 20.0, 0.3)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    ref, _ = W.worker_search(w.search_args(), series, w.dm_list(), accs, forest=True)
+    got, _ = W.batch_search(w.search_args(), series, w.dm_list(), accs, W.make_batch_opts(16), forest=True)
+    assert len(ref) > 100
+    assert same(got, ref)
+
+
 def test_async_prologue_baseline_loop_matches_worker(psx):
     """The batched search runs the DM prologues ahead on their own stream
     with a fixed number of device iterations of remove_baseline's loop
