@@ -119,7 +119,11 @@ def serve_stream(P, W, nreq=32, rate=4.0, seed=0x5EA505, budget=200e9, batch=16,
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(w.search_args(), d, w.dm_list(), nsamps=w.size))
-        sts = [srv.wait(i, timeout=600) for i in ids]
+        sts = []
+        for k, i in enumerate(ids):
+            sts.append(srv.wait(i, timeout=600))
+            if (k + 1) % 32 == 0 and k + 1 < len(ids):
+                log("serving: %d of %d requests done" % (k + 1, len(ids)))
         wall = time.perf_counter() - t0
         for i in ids:
             srv.result(i)
