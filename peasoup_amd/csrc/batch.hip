@@ -109,7 +109,8 @@ __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, 
 // for the interbin neighbours: P(q, e) uses X(q-1, e), and P(R-q', L-1-e)
 // uses X(R-q'-1, L-1-e), the mirror of pair row q'+1.  Results are
 // transposed through LDS and written as 64-bin runs of P (natural order).
-// Rows 0 and R/2 and bin M pair within their own row: k_spec_edge.
+// Rows 0 and R/2 and bin M pair within their own row: the grid's last row
+// of workgroups computes them (spec_edge_bin), beside the tiles.
 constexpr int kRowTile = 64;  // lower rows per workgroup (16 per wave)
 constexpr int kColTile = 64;  // columns per workgroup (one per lane)
 // the write-out gives each lane one row of the tile
@@ -126,6 +127,36 @@ __device__ inline void untangle_pair(float2 zk, float2 zm, uint32_t k, uint32_t 
   xm = untangle_one(zm, zk, upper ? w : wm);
 }
 
+// Bins of rows 0 and R/2 (k = R*e and R/2 + R*e) and the Nyquist bin M of
+// the transposed layout, one thread each, through the natural-index view.
+__device__ inline float2 z_rows(const float2* y, uint32_t R, uint32_t L, uint32_t j) {
+  const uint32_t M = R * L;
+  if (j >= M) j -= M;
+  return y[(j % R) * L + j / R];
+}
+__device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_t k, const RootTable& rt) {
+  const uint32_t M = R * L;
+  float2 xk, xm;
+  untangle_pair(z_rows(y, R, L, k), z_rows(y, R, L, M - k), k, M, rt, xk, xm);
+  return xk;
+}
+// t < L: row 0; t < 2L: row R/2; t == 2L: bin M
+__device__ inline void spec_edge_bin(const float2* y, float* p, uint32_t R, uint32_t L, uint32_t t,
+                                     const RootTable& rt, float mean_n, float std_n) {
+  const uint32_t M = R * L;
+  const uint32_t k = t < L ? R * t : (t < 2 * L ? R / 2 + R * (t - L) : M);
+  float2 x, prev = make_float2(0.f, 0.f);
+  if (k == M) {
+    // X[M] is the upper bin of pair 0
+    float2 x0;
+    untangle_pair(y[0], y[0], 0, M, rt, x0, x);
+  } else {
+    x = x_rows(y, R, L, k, rt);
+  }
+  if (k > 0) prev = x_rows(y, R, L, k - 1, rt);
+  p[k] = normalised(x, prev, mean_n, std_n);
+}
+
 __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
                                                   uint32_t L, int64_t pstride, RootTable rt,
                                                   const TrialDesc* __restrict__ descs) {
@@ -138,6 +169,12 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   const float2* y = Y + (uint64_t)tr * M;
   float* p = P + (int64_t)tr * pstride;
   const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
+  if (blockIdx.y == gridDim.y - 1) {
+    // the edge bins: L/64 workgroups of kB threads cover the 2L+1 of them
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t <= 2 * L) spec_edge_bin(y, p, R, L, t, rt, mean_n, std_n);
+    return;
+  }
   const uint32_t ra = 1 + blockIdx.y * kRowTile, ea = blockIdx.x * kColTile;
   const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint32_t e = ea + lane, emir = L - 1 - e;
@@ -192,44 +229,6 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
       stg1(p, r + R * (ea + j), lo_t[j][rr]);
       stg1(p, (R - r) + R * (L - 1 - (ea + j)), up_t[j][rr]);
     }
-  }
-}
-
-// Bins of rows 0 and R/2 (k = R*e and R/2 + R*e) and the Nyquist bin M of
-// the transposed layout, one thread each, through the natural-index view.
-__device__ inline float2 z_rows(const float2* y, uint32_t R, uint32_t L, uint32_t j) {
-  const uint32_t M = R * L;
-  if (j >= M) j -= M;
-  return y[(j % R) * L + j / R];
-}
-__device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_t k, const RootTable& rt) {
-  const uint32_t M = R * L;
-  float2 xk, xm;
-  untangle_pair(z_rows(y, R, L, k), z_rows(y, R, L, M - k), k, M, rt, xk, xm);
-  return xk;
-}
-
-__global__ void __launch_bounds__(kB) k_spec_edge(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
-                                                  uint32_t L, int64_t pstride, RootTable rt,
-                                                  const TrialDesc* __restrict__ descs) {
-  const uint32_t tr = blockIdx.y;
-  const uint32_t M = R * L;
-  const float2* y = Y + (uint64_t)tr * M;
-  float* p = P + (int64_t)tr * pstride;
-  const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
-  for (uint32_t t = blockIdx.x * kB + threadIdx.x; t <= 2 * L; t += gridDim.x * kB) {
-    // t < L: row 0; t < 2L: row R/2; t == 2L: bin M
-    const uint32_t k = t < L ? R * t : (t < 2 * L ? R / 2 + R * (t - L) : M);
-    float2 x, prev = make_float2(0.f, 0.f);
-    if (k == M) {
-      // X[M] is the upper bin of pair 0
-      float2 x0;
-      untangle_pair(y[0], y[0], 0, M, rt, x0, x);
-    } else {
-      x = x_rows(y, R, L, k, rt);
-    }
-    if (k > 0) prev = x_rows(y, R, L, k - 1, rt);
-    p[k] = normalised(x, prev, mean_n, std_n);
   }
 }
 
@@ -377,13 +376,11 @@ int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t 
                      const TrialDesc* d_descs, uint32_t ntr, hipStream_t s) {
   if (R < 4 || R % 2 || L % kColTile || (uint64_t)R * L > 0x7FFFFFFFull || pstride < (int64_t)R * L + 1)
     return psx::fail(PSX_ERR_ARG, "spec_rows: unsupported layout %u x %u", R, L);
+  // row tiles of rows [1, R/2), then one row of workgroups for the edge bins
   const unsigned row_tiles = (R / 2 - 1 + kRowTile - 1) / kRowTile;
-  if (row_tiles > 0) {
-    k_spec_rows<<<dim3(L / kColTile, row_tiles, ntr), kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
-    PSX_CHECK_LAUNCH("k_spec_rows");
-  }
-  k_spec_edge<<<dim3(psx::grid_for(2 * (uint64_t)L + 1, kB, 64), ntr), kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
-  PSX_CHECK_LAUNCH("k_spec_edge");
+  static_assert(kColTile * 2 <= kB, "the edge row needs 2L+1 threads from L/kColTile workgroups");
+  k_spec_rows<<<dim3(L / kColTile, row_tiles + 1, ntr), kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+  PSX_CHECK_LAUNCH("k_spec_rows");
   return PSX_OK;
 }
 

@@ -96,7 +96,8 @@ def stage_of(name):
     if "k_fft_pass<" in name:
         return "fft_pass0_resample" if re.search(r",\s*true>\s*$", name) else "fft_pass"
     # the spectrum pass: natural layout (k_spec_post) or the transposed row
-    # layout (k_spec_rows; its small companion k_spec_edge is not counted)
+    # layout (k_spec_rows, whose last row of workgroups also computes the
+    # layout's edge bins)
     if "k_spec_post" in name or "k_spec_rows" in name:
         return "spec_post"
     if "k_harm_peaks" in name:
