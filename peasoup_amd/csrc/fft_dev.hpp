@@ -122,6 +122,12 @@ __device__ inline float2 ldg2(const float2* base, uint32_t i) {
 __device__ inline float ldg1(const float* base, uint32_t i) {
   return *(const PSX_GLOBAL float*)((const PSX_GLOBAL char*)base + (uint64_t)(i * 4u));
 }
+// floats i and i+1 with one 8-byte load that needs only 4-byte alignment
+typedef float psx_f2u __attribute__((ext_vector_type(2), aligned(4)));
+__device__ inline float2 ldg2u(const float* base, uint32_t i) {
+  const psx_f2u v = *(const PSX_GLOBAL psx_f2u*)((const PSX_GLOBAL char*)base + (uint64_t)(i * 4u));
+  return make_float2(v.x, v.y);
+}
 __device__ inline void stg1(float* base, uint32_t i, float v) {
   *(PSX_GLOBAL float*)((PSX_GLOBAL char*)base + (uint64_t)(i * 4u)) = v;
 }
