@@ -244,16 +244,33 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
     }
   } else {
     const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)blockIdx.z * a.in_tr;
+    auto load = [&](unsigned cbase, float2(&dst)[P]) {
 #pragma unroll
-    for (int b = 0; b < NB0; ++b) {
-      int c, j;
-      bf_coords<L, C, R0, NT>(b, t, cf, c, j);
-      const uint32_t o0 = (col0 + (uint32_t)c) * icol + (uint32_t)j * iel;
+      for (int b = 0; b < NB0; ++b) {
+        int c, j;
+        bf_coords<L, C, R0, NT>(b, t, cf, c, j);
+        const uint32_t o0 = (cbase + (uint32_t)c) * icol + (uint32_t)j * iel;
 #pragma unroll
-      for (int r = 0; r < R0; ++r) {
-        float2 x = ldg2(in, o0 + (uint32_t)(r * (L / R0)) * iel);
-        if (a.in_conj) x.y = -x.y;
-        v[b * R0 + r] = x;
+        for (int r = 0; r < R0; ++r) {
+          float2 x = ldg2(in, o0 + (uint32_t)(r * (L / R0)) * iel);
+          if (a.in_conj) x.y = -x.y;
+          dst[b * R0 + r] = x;
+        }
+      }
+    };
+    load(col0, v);
+    if constexpr (two_tiles_ok(L, C, MODE)) {
+      const unsigned blk2 = blk + gridDim.x;
+      if (a.tiles2 && blk2 < a.ntiles) {
+        // the second tile's loads are in flight while the first is
+        // transformed; the barrier keeps the first tile's last LDS reads
+        // ahead of the second's first writes
+        float2 v2[P];
+        load(blk2 * C, v2);
+        run_stage<0, L, C, MODE>(v, tw, lds, a, out, col0, t);
+        __syncthreads();
+        run_stage<0, L, C, MODE>(v2, tw, lds, a, out, blk2 * C, t);
+        return;
       }
     }
   }
@@ -318,12 +335,13 @@ struct KernelInfo {
   void (*fn)(const PassArgs);
   int threads;
   int lds;
+  bool two_tiles;  // may run two column tiles per workgroup (PassArgs::tiles2)
 };
 
 template <int L, int C, int MODE, bool RS>
 constexpr KernelInfo kinfo1() {
   return KernelInfo{L, C, MODE, RS, &k_fft_pass<L, C, MODE, RS>, C * L / points_per_thread(L),
-                    pass_lds_bytes(L, C, MODE)};
+                    pass_lds_bytes(L, C, MODE), !RS && two_tiles_ok(L, C, MODE)};
 }
 // single-pass kernels (C = 1): contiguous, plain and resampling
 #define PSX_K1(L) kinfo1<L, 1, PM_CONTIG, false>(), kinfo1<L, 1, PM_CONTIG, true>()
@@ -794,7 +812,13 @@ int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t n
   if (rc) return rc;
   const KernelInfo* k = q == 0 ? L.krs : L.k;
   if (!k) return psx::fail(PSX_ERR_UNSUPPORTED, "exec_fused_pass: no resampling kernel for this pass");
-  hipLaunchKernelGGL(k->fn, dim3(L.gx, L.gy, ntrials), dim3(k->threads), k->lds, hs, a);
+  unsigned gx = L.gx;
+  if (k->two_tiles && gx > 1) {
+    a.tiles2 = 1;
+    a.ntiles = gx;
+    gx = (gx + 1) / 2;
+  }
+  hipLaunchKernelGGL(k->fn, dim3(gx, L.gy, ntrials), dim3(k->threads), k->lds, hs, a);
   PSX_CHECK_LAUNCH("k_fft_pass(fused)");
   return PSX_OK;
 }

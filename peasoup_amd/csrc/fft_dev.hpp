@@ -295,7 +295,18 @@ struct PassArgs {
   int32_t resample;
   uint64_t rs_n;
   const TrialDesc* descs;
+  // Two column tiles per workgroup (kernels where two_tiles_ok): the grid
+  // covers ceil(ntiles/2) tiles and workgroup b also does tile b + gridDim.x
+  // (< ntiles), its loads issued before the first tile is transformed.
+  int32_t tiles2;
+  uint32_t ntiles;
 };
+// Passes that may take two tiles per workgroup: contiguous tiles of at
+// least 4 rows whose register budget (at least 128 VGPRs) holds a second
+// tile's points while the first is transformed (narrower tiles spill).
+__host__ __device__ constexpr bool two_tiles_ok(int L, int C, int mode) {
+  return mode == 0 && C >= 4 && points_per_thread(L) <= 16 && waves_per_simd(L, C, mode) <= 4;
+}
 
 __device__ inline float2 tw4_lookup(const PassArgs& a, uint64_t x) {
   const float2 lo = a.tw4_lo[x & a.tw4_mask];
