@@ -118,10 +118,13 @@ static_assert(kRowTile == 64 && kColTile == 64, "k_spec_rows writes one tile row
 
 // X[k] of the pair (Z[k], Z[M-k]) exactly as untangled_bin computes it:
 // the root of min(k, M-k), mirrored for the upper bin.
+// hi (optional): the root table's high factor of this root, when the
+// caller already holds it; the product is the same cmul either way.
 __device__ inline void untangle_pair(float2 zk, float2 zm, uint32_t k, uint32_t M, const RootTable& rt, float2& xk,
-                                     float2& xm) {
+                                     float2& xm, const float2* hi = nullptr) {
   const bool upper = 2 * k > M;
-  const float2 w = root32(rt, upper ? M - k : k);
+  const uint32_t ri = upper ? M - k : k;
+  const float2 w = hi ? cmul(*hi, ldg2(rt.lo, ri & rt.mask)) : root32(rt, ri);
   const float2 wm = make_float2(-w.x, w.y);
   xk = untangle_one(zk, zm, upper ? wm : w);
   xm = untangle_one(zm, zk, upper ? w : wm);
@@ -157,6 +160,7 @@ __device__ inline void spec_edge_bin(const float2* y, float* p, uint32_t R, uint
   p[k] = normalised(x, prev, mean_n, std_n);
 }
 
+template <bool HI_FIXED>
 __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
                                                   uint32_t L, int64_t pstride, RootTable rt,
                                                   const TrialDesc* __restrict__ descs) {
@@ -187,6 +191,21 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
     const uint32_t ih = q == 0 ? (e == 0 ? 0 : L - e) : (R - q) * L + emir;
     untangle_pair(ldg2(y, q * L + e), ldg2(y, ih), q + R * e, M, rt, xlo, xhi);
   };
+  // HI_FIXED (R a power of two no larger than the root table's low span
+  // 2^shift, checked by the launcher): for rows q in [1, R/2] of one column
+  // the root index is q + R*e or its mirror R*(L-e) - q, whose high part
+  // (index >> shift) is then the same for all of them, so the sweep loads
+  // that factor once per lane.
+  float2 hi_e = make_float2(0.f, 0.f);
+  if constexpr (HI_FIXED) {
+    const uint32_t k0 = min(r0, half_rows) + R * e;
+    hi_e = ldg2(rt.hi, (2 * k0 > M ? M - k0 : k0) >> rt.shift);
+  }
+  auto sweep_row = [&](uint32_t qq, float2& xlo, float2& xhi) {
+    const uint32_t q = min(qq, half_rows);  // >= 1 in the sweep
+    untangle_pair(ldg2(y, q * L + e), ldg2(y, (R - q) * L + emir), q + R * e, M, rt, xlo, xhi,
+                  HI_FIXED ? &hi_e : nullptr);
+  };
   // results of rows outside [1, R/2) go to the spare slot kRowTile
   auto slot = [&](uint32_t qq) { return qq < half_rows ? qq - ra : (uint32_t)kRowTile; };
   // Branch-free sweep over the wave's own rows (so the loads of later rows
@@ -199,7 +218,7 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   for (int i = 0; i < kWaveRows; ++i) {
     const uint32_t qq = r0 + (uint32_t)i;
     float2 xlo, xhi;
-    pair_row(qq, xlo, xhi);
+    sweep_row(qq, xlo, xhi);
     if (i == 0) {
       xlo_first = xlo;
       edge_hi[wv][lane] = xhi;
@@ -398,7 +417,11 @@ int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t 
   // row tiles of rows [1, R/2), then one row of workgroups for the edge bins
   const unsigned row_tiles = (R / 2 - 1 + kRowTile - 1) / kRowTile;
   static_assert(kColTile * 2 <= kB, "the edge row needs 2L+1 threads from L/kColTile workgroups");
-  k_spec_rows<<<dim3(L / kColTile, row_tiles + 1, ntr), kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+  const dim3 grid(L / kColTile, row_tiles + 1, ntr);
+  if ((R & (R - 1)) == 0 && R <= (1u << rt.shift))
+    k_spec_rows<true><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+  else
+    k_spec_rows<false><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
   PSX_CHECK_LAUNCH("k_spec_rows");
   return PSX_OK;
 }
