@@ -163,18 +163,42 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2 (&tw)
   } else {
     // element offsets inside one transform are < n/2 <= 2^27: 32-bit math
     const uint32_t ocol = (uint32_t)a.out_col, oel = (uint32_t)a.out_elem;
+    // four-step twiddle W_Mtw^(cg*e) with e = j + r*(L/R): for a last radix
+    // R <= 8, the factor W_Mtw^(cg*j) from the two-level table once per
+    // butterfly, times W_Mtw^(cg*r*L/R) = W_(Mtw*R/L)^(cg*r), which depends
+    // on the column only (the column-fast mapping keeps a thread's column
+    // for every b): R - 1 loads per thread instead of two per point.  A
+    // radix-16 last stage would hold 16 such factors and spill at the
+    // 64-VGPR budget: it looks up each point's twiddle whole.
+    constexpr bool kSplitTw = ModeTraits<MODE>::tw4 && R <= 8;
+    float2 wr[R];
+    if constexpr (kSplitTw) {
+      static_assert(cf, "the four-step twiddle needs the column-fast output mapping");
+      int c0, j0;
+      bf_coords<L, C, R, NT>(0, t, cf, c0, j0);
+      const uint32_t cg0 = col0 + (uint32_t)c0;
+      wr[0] = make_float2(1.f, 0.f);
+#pragma unroll
+      for (int r = 1; r < R; ++r) wr[r] = ldg2(a.tw4_r, cg0 * (uint32_t)r);
+    }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       int c, j;
       bf_coords<L, C, R, NT>(b, t, cf, c, j);
       const uint32_t cg = col0 + (uint32_t)c;
       const uint32_t o0 = cg * ocol + (uint32_t)j * oel;
+      float2 wj = make_float2(1.f, 0.f);
+      if constexpr (kSplitTw) {
+        const uint32_t x = cg * (uint32_t)j;
+        wj = cmul(ldg2(a.tw4_hi, x >> a.tw4_shift), ldg2(a.tw4_lo, x & a.tw4_mask));
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const uint32_t e = (uint32_t)(j + r * (L / R));
         float2 val = v[b * R + r];
-        if constexpr (ModeTraits<MODE>::tw4) {
-          const uint32_t x = cg * e;
+        if constexpr (kSplitTw) {
+          val = cmul(val, r == 0 ? wj : cmul(wj, wr[r]));
+        } else if constexpr (ModeTraits<MODE>::tw4) {
+          const uint32_t x = cg * (uint32_t)(j + r * (L / R));
           val = cmul(val, cmul(ldg2(a.tw4_hi, x >> a.tw4_shift), ldg2(a.tw4_lo, x & a.tw4_mask)));
         }
         if (a.out_conj) val.y = -val.y;
@@ -559,7 +583,7 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
   const uint64_t M = p->M;
   // --- device tables: per-pass W_L, four-step roots, W_n roots
   std::vector<float2> host;
-  std::vector<size_t> twl_off(d.npass), tw4lo_off(d.npass), tw4hi_off(d.npass);
+  std::vector<size_t> twl_off(d.npass), tw4lo_off(d.npass), tw4hi_off(d.npass), tw4r_off(d.npass);
   std::vector<uint32_t> tw4_shift(d.npass);
   auto append = [&](const std::vector<float2>& v) {
     size_t o = host.size();
@@ -585,6 +609,14 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
       root_table_host(mtw[q], sh, lo, hi);
       tw4lo_off[q] = append(lo);
       tw4hi_off[q] = append(hi);
+      // W_T^k, T = Mtw * R / L with R the pass's last radix (the output
+      // twiddle's column factor)
+      const uint64_t T = mtw[q] * (uint64_t)psx::fft::radix_at(d.L[q], psx::fft::num_stages(d.L[q]) - 1) /
+                         (uint64_t)d.L[q];
+      tmp.resize(T);
+      const long double tpr = -2.0L * 3.14159265358979323846264338327950288L / (long double)T;
+      for (uint64_t k = 0; k < T; ++k) tmp[k] = make_float2((float)cosl(tpr * k), (float)sinl(tpr * k));
+      tw4r_off[q] = append(tmp);
     }
   }
   uint32_t rsh = 0;
@@ -615,6 +647,7 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
       a.tw4 = 1;
       a.tw4_lo = p->tables + tw4lo_off[q];
       a.tw4_hi = p->tables + tw4hi_off[q];
+      a.tw4_r = p->tables + tw4r_off[q];
       a.tw4_shift = tw4_shift[q];
       a.tw4_mask = (uint32_t)((1ull << tw4_shift[q]) - 1);
     }

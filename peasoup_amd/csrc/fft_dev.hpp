@@ -284,6 +284,7 @@ struct PassArgs {
   const float2* tw4_hi;  // W_Mtw^((x >> shift) << shift)
   uint32_t tw4_shift;
   uint32_t tw4_mask;
+  const float2* tw4_r;  // W_(Mtw*R/L)^k, R the last stage's radix
   int32_t tw4;       // multiply output (c, e) by W_Mtw^(c*e)
   int32_t in_cf;     // stage-0 mapping column-fast
   int32_t out_cf;    // last-stage mapping column-fast
