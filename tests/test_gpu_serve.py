@@ -132,4 +132,8 @@ def test_tuned_schedule_gives_same_results(psx, reqs):
     got = run({w.size: {"batch": 3}})
     assert same(got, ref) and len(got) == len(ref)
     got = run({w.size: {"batch": 3, "decomp": [(64, 16), (1024, 8)]}})
+    assert len(ref) > 0 and len(got) > 0
     assert not compare(got, ref, w.min_snr, check_assoc=False)
+    # the injected 5.3 ms pulsar of DM trial 1 is among them
+    assert np.any((got["dm_idx"] == 1) & (got["snr"] > 10) &
+                  (np.abs(got["freq"] * 5.3e-3 - np.round(got["freq"] * 5.3e-3)) < 0.01))

@@ -163,7 +163,10 @@ def test_median_scrunch5(psx, orc, count):
     assert np.array_equal(d_o.numpy()[:ref.size], ref)
 
 
-@pytest.mark.parametrize("inc,outc", [(2, 10), (13107, 65537), (838860, (1 << 22) + 1), (33554, (1 << 22) + 1)])
+# the last two stretch to 2^25 + 1 outputs (2^26 samples), where the float
+# index x = i*step is inexact for i >= 2^24 (src/kernels.cu:1019-1020)
+@pytest.mark.parametrize("inc,outc", [(2, 10), (13107, 65537), (838860, (1 << 22) + 1), (33554, (1 << 22) + 1),
+                                      (6710886, (1 << 25) + 1), (268435, (1 << 25) + 1)])
 def test_linear_stretch(psx, orc, inc, outc):
     x = RNG.standard_normal(inc).astype(np.float32)
     d_x, d_o = dev(psx, x), out(psx, outc)
@@ -171,7 +174,7 @@ def test_linear_stretch(psx, orc, inc, outc):
     assert np.array_equal(d_o.numpy(), orc.linear_stretch(x, outc))
 
 
-@pytest.mark.parametrize("n,tsamp", [(1 << 17, 64e-6), (1 << 23, 76e-6), (3 * (1 << 21), 76e-6)])
+@pytest.mark.parametrize("n,tsamp", [(1 << 17, 64e-6), (1 << 23, 76e-6), (3 * (1 << 21), 76e-6), (1 << 26, 64e-6)])
 def test_deredden_median(psx, orc, n, tsamp):
     nb = n // 2 + 1
     A = np.abs(RNG.standard_normal(nb)).astype(np.float32) * 10
