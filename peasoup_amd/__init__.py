@@ -176,6 +176,10 @@ class DeviceArray:
     def offset(self, elems):
         return self.ptr + int(elems) * self.dtype.itemsize
 
+    def view(self, elem_offset, shape):
+        """Non-owning window of `shape` elements starting at `elem_offset`."""
+        return DeviceView(self, elem_offset, shape)
+
     def free(self):
         if getattr(self, "ptr", None):
             lib().psx_free(self.ptr)
@@ -186,6 +190,24 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView(DeviceArray):
+    """Window into a DeviceArray; keeps its owner alive and frees nothing."""
+
+    def __init__(self, owner, elem_offset, shape):
+        self.owner = owner
+        self.dtype = owner.dtype
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
+        if elem_offset < 0 or elem_offset + self.size > owner.size:
+            raise ValueError("view [%d, %d) outside a buffer of %d elements" %
+                             (elem_offset, elem_offset + self.size, owner.size))
+        self.ptr = owner.offset(elem_offset)
+
+    def free(self):
+        self.ptr = None
 
 
 def synchronize():

@@ -356,11 +356,35 @@ inline void stats_with(float* ptr, unsigned int nsamps, float* mean_, float* rms
   *std_ = std::sqrt(r * r - m * m);
 }
 
+// Reduction scratch of one call (the reference's Thrust reductions allocate
+// theirs per call as well, kernels.cu:440-455).
+struct ReduceWork {
+  void* p = nullptr;
+  ReduceWork() { peasoup::check(psx_malloc(&p, psx_reduce_work_bytes()), "stats"); }
+  ~ReduceWork() { psx_free(p); }
+  ReduceWork(const ReduceWork&) = delete;
+  ReduceWork& operator=(const ReduceWork&) = delete;
+};
+
 template <class T>
 void stats(T* ptr, unsigned int nsamps, float* mean_, float* rms_, float* std_, unsigned int first_samp = 0) {
-  static thread_local void* work = nullptr;
-  if (!work) peasoup::check(psx_malloc(&work, psx_reduce_work_bytes()), "stats");
-  stats_with(ptr, nsamps, mean_, rms_, std_, work, first_samp);
+  ReduceWork w;
+  stats_with(ptr, nsamps, mean_, rms_, std_, w.p, first_samp);
+}
+
+// stats.hpp:9-19: rms and mean of samples [first_samp, nsamps)
+template <class T>
+float rms(T* ptr, unsigned int nsamps, unsigned int first_samp = 0) {
+  float m, r, sd;
+  stats(ptr, nsamps, &m, &r, &sd, first_samp);
+  return r;
+}
+
+template <class T>
+float mean(T* ptr, unsigned int nsamps, unsigned int first_samp = 0) {
+  float m, r, sd;
+  stats(ptr, nsamps, &m, &r, &sd, first_samp);
+  return m;
 }
 
 inline float std(float mean, float rms) { return std::sqrt(rms * rms - mean * mean); }

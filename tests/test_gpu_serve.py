@@ -137,3 +137,54 @@ def test_tuned_schedule_gives_same_results(psx, reqs):
     # the injected 5.3 ms pulsar of DM trial 1 is among them
     assert np.any((got["dm_idx"] == 1) & (got["snr"] > 10) &
                   (np.abs(got["freq"] * 5.3e-3 - np.round(got["freq"] * 5.3e-3)) < 0.01))
+
+
+def test_serving_sizes_alone_and_in_the_mix(psx):
+    """Serving-configuration sizes: beams of 16 DM trials at 2^20, 2^22 and
+    2^24 samples (64 us, red noise, a pulsar in each), a few accelerations
+    each, under the tuned per-length schedules.  Each request's candidates
+    are the same alone and in the mix, and equal the batched search of the
+    same series with the same schedule."""
+    import peasoup_amd.workloads as W
+    from peasoup_amd import tuning
+    from peasoup_amd.serve import Server
+    table = tuning.table(psx.device_info(0)["name"])
+    specs = []
+    for k, n in enumerate((1 << 20, 1 << 22, 1 << 24)):
+        w = W.Workload("beam%d" % n, 16, n, 64e-6, 0x5EA5D0 + k, 5.0, -50.0, 50.0)
+        accs_all = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+        j = int(0.62 * (accs_all.size - 1))
+        w.injections = [W.Injection(3 + 5 * k, 0.0047 * (k + 1), float(accs_all[j]), 20.0, 0.3)]
+        accs = accs_all[sorted({0, j, accs_all.size - 1})]
+        specs.append((w, W.generate_device(w), accs))
+    psx.synchronize()
+    direct = []
+    for w, d, accs in specs:
+        t = table.get(w.size, {})
+        opts = W.make_batch_opts(t.get("batch") or 16, decomp=t.get("decomp"))
+        direct.append(W.batch_search(w.search_args(), d, w.dm_list(), accs, opts, nsamps=w.size)[0])
+        print("beam %d: %d candidates" % (w.size, len(direct[-1])), flush=True)
+    alone = []
+    for w, d, accs in specs:
+        with Server(200e9, batch=16, schedules=table) as srv:
+            rid = srv.submit(w.search_args(), d, w.dm_list(), accs=accs, nsamps=w.size)
+            assert srv.wait(rid)["state"] == "done"
+            alone.append(srv.result(rid))
+    with Server(200e9, batch=16, schedules=table) as srv:
+        # the largest first, so the others arrive while it runs
+        ids = {}
+        for k in (2, 0, 1):
+            w, d, accs = specs[k]
+            ids[k] = srv.submit(w.search_args(), d, w.dm_list(), accs=accs, nsamps=w.size)
+        sts = {k: srv.wait(i) for k, i in ids.items()}
+        assert all(s["state"] == "done" for s in sts.values()), sts
+        mixed = {k: srv.result(i) for k, i in ids.items()}
+        assert srv.stats()["engines"] == 3
+    for k, (w, _, _) in enumerate(specs):
+        assert len(direct[k]) > 0
+        assert same(alone[k], direct[k]), w.name
+        assert same(mixed[k], alone[k]), w.name
+        inj = w.injections[0]
+        c = direct[k][direct[k]["dm_idx"] == inj.trial]
+        r = c["freq"] * inj.period
+        assert np.any((c["snr"] > 9) & (np.abs(r - np.round(r)) < 0.01)), (w.name, c[:5])
