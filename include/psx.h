@@ -110,6 +110,18 @@ int psx_zap_birdies(psx_c32* X, const float* freqs, const float* widths, uint32_
 uint64_t psx_deredden_work_floats(uint64_t nb);
 int psx_deredden_median(const float* A, uint64_t nb, double bin_width, float boundary_5_freq,
                         float boundary_25_freq, float* median, float* work, psx_stream s);
+/* No counterpart in the reference: psx_deredden_median for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  It allocates nothing: the
+ * scrunched medians live in a static device array of
+ * psx_deredden_work_floats(psx_deredden_noalloc_max_bins()) floats (33 MB)
+ * held by each device until the process ends.  nb above
+ * psx_deredden_noalloc_max_bins() (2^25 + 1, a 2^26-sample series) is
+ * rejected with PSX_ERR_ARG.  Calls share that array per device, so calls
+ * that overlap on one device race; the results equal psx_deredden_median's. */
+uint64_t psx_deredden_noalloc_max_bins(void);
+int psx_deredden_median_noalloc(const float* A, uint64_t nb, double bin_width, float boundary_5_freq,
+                                float boundary_25_freq, float* median, psx_stream s);
 
 /* GPU_mean / GPU_rms (src/kernels.cu:427-461) as used by stats::stats
  * (stats.hpp:25-33).  Synchronises s; results on the host.  work must hold

@@ -8,6 +8,7 @@
 // multiply-adds that the reference's nvcc build contracts are written as
 // explicit fma so that results match the CPU restatement bit for bit.
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 #include "kernels_dev.hpp"
@@ -512,6 +513,53 @@ extern "C" int psx_deredden_median(const float* A, uint64_t nb, double bin_width
     PSX_CHECK_LAUNCH("k_stretch_zone");
   }
   return psx::ok();
+}
+
+// ------------------------------------------------ running median, no allocation
+// psx_deredden_median_noalloc: the same kernels, with the three scrunched
+// medians kept in a file-scope device array instead of a caller workspace.
+// The array's name is only a host-side handle: kernels get its device
+// address, looked up once per device with hipGetSymbolAddress.  Every call on
+// a device shares that device's array (calls that overlap on one device
+// race); every element a call reads is written earlier in the same call, so
+// nothing needs clearing.
+namespace {
+constexpr uint64_t kNoAllocMaxBins = (1ull << 25) + 1;  // 2^26-sample series
+constexpr uint64_t kNoAllocScratchFloats = kNoAllocMaxBins / 5 + kNoAllocMaxBins / 25 + kNoAllocMaxBins / 125 + 3;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+static int scratch_on_current_device(float** out) {
+  int dev = 0;
+  PSX_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return psx::fail(PSX_ERR_UNSUPPORTED, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return psx::fail(PSX_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (float*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PSX_OK;
+}
+
+extern "C" uint64_t psx_deredden_noalloc_max_bins(void) { return kNoAllocMaxBins; }
+
+extern "C" int psx_deredden_median_noalloc(const float* A, uint64_t nb, double bin_width, float b5, float b25,
+                                           float* median, psx_stream s) {
+  PSX_REQUIRE(nb <= kNoAllocMaxBins,
+              "psx_deredden_median_noalloc: %llu bins exceed the static scratch (at most %llu bins)",
+              (unsigned long long)nb, (unsigned long long)kNoAllocMaxBins);
+  float* work = nullptr;
+  const int rc = scratch_on_current_device(&work);
+  if (rc) return rc;
+  return psx_deredden_median(A, nb, bin_width, b5, b25, median, work, s);
 }
 
 extern "C" uint64_t psx_reduce_work_bytes(void) { return (kReduceBlocks + 1) * sizeof(double2); }
