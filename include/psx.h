@@ -160,6 +160,13 @@ enum { PSX_FFT_R2C = 0, PSX_FFT_C2R = 1 };
 /* Allocating: the plan owns its twiddle tables and work buffer.  n is the real
  * length (even; 2^k or 3*2^k, 2^4 <= n <= 2^28). */
 int psx_fft_plan_create(uint64_t n, int kind, uint32_t batch, psx_fft_plan** plan);
+/* The same plan with a chosen decomposition: npass (1-3) passes of lengths
+ * lens[q] (product n/2; pass 0 strided, the last contiguous) and column
+ * tiles cols[q] (0 = the widest that fits).  cuFFT picks its own
+ * decomposition (ffter.hpp:60-70 takes none); this is how the autotuner's
+ * schedules (psx_batch_opts.fft_len/fft_cols) are checked as plain FFTs. */
+int psx_fft_plan_create_decomp(uint64_t n, int kind, uint32_t batch, int npass, const int* lens, const int* cols,
+                               psx_fft_plan** plan);
 int psx_fft_plan_destroy(psx_fft_plan* plan);
 /* R2C: in = batch x n float, out = batch x (n/2+1) psx_c32 (unnormalised).
  * C2R: in = batch x (n/2+1) psx_c32, out = batch x n float (scaled by n).

@@ -87,6 +87,7 @@ def lib():
         "psx_find_peaks_work_bytes": (u64, [u64]),
         "psx_find_peaks": (i32, [vp, i64, i64, f32, vp, vp, u32, vp, vp, vp]),
         "psx_fft_plan_create": (i32, [u64, i32, u32, C.POINTER(vp)]),
+        "psx_fft_plan_create_decomp": (i32, [u64, i32, u32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(vp)]),
         "psx_fft_plan_destroy": (i32, [vp]),
         "psx_fft_exec": (i32, [vp, vp, vp, vp]),
         "psx_fft_plan_describe": (i32, [vp, C.c_char_p, C.c_size_t]),
@@ -258,9 +259,16 @@ class Event:
 class FFTPlan:
     """R2C / C2R plan (CuFFTerR2C / CuFFTerC2R, ffter.hpp:54-110)."""
 
-    def __init__(self, n, kind=PSX_FFT_R2C, batch=1):
+    def __init__(self, n, kind=PSX_FFT_R2C, batch=1, decomp=None):
+        """decomp: optional [(length, column tile), ...] per pass (the
+        autotuner's schedule form); None picks the default."""
         p = C.c_void_p()
-        check(lib().psx_fft_plan_create(n, kind, batch, C.byref(p)))
+        if decomp:
+            lens = (C.c_int * 3)(*[int(L) for L, _ in decomp])
+            cols = (C.c_int * 3)(*[int(c or 0) for _, c in decomp])
+            check(lib().psx_fft_plan_create_decomp(n, kind, batch, len(decomp), lens, cols, C.byref(p)))
+        else:
+            check(lib().psx_fft_plan_create(n, kind, batch, C.byref(p)))
         self.handle = p.value
         self.n, self.kind, self.batch = n, kind, batch
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "trace.hpp"
 #include "psx.h"
 
 namespace {
@@ -140,6 +141,7 @@ extern "C" int psx_dedisperse(const uint8_t* d_in, uint64_t nsamps_in, uint32_t 
                               const float* dm_list, uint32_t ndm, const int32_t* killmask, float tsamp, float f0,
                               float df, float* d_out, uint64_t out_stride, void* d_work, psx_stream s) {
   PSX_REQUIRE(d_in && dm_list && d_out && d_work && ndm > 0 && nsamps_in > 0, "psx_dedisperse: bad arguments");
+  psx::TraceRange range("Dedisperse");
   PSX_REQUIRE(nchans > 0 && nchans <= kMaxChans, "psx_dedisperse: nchans must be 1..%u", kMaxChans);
   PSX_REQUIRE(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8, "psx_dedisperse: nbits must be 1, 2, 4 or 8");
   PSX_REQUIRE((uint64_t)nchans * nbits % 8 == 0, "psx_dedisperse: a sample must fill whole bytes");

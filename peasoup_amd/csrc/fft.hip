@@ -920,6 +920,18 @@ extern "C" int psx_fft_plan_describe(const psx_fft_plan* p, char* buf, size_t ca
 // Host-only geometry check of a plan: builds the pass launches without device
 // tables and validates every pass's largest read and write index against its
 // buffers, for the plain transform and for the fused batched use.
+extern "C" int psx_fft_plan_create_decomp(uint64_t n, int kind, uint32_t batch, int npass, const int* lens,
+                                          const int* cols, psx_fft_plan** plan) {
+  PSX_REQUIRE(npass >= 1 && npass <= 3 && lens && cols, "psx_fft_plan_create_decomp: bad decomposition");
+  psx::fft::Decomp d;
+  d.npass = npass;
+  for (int q = 0; q < npass; ++q) {
+    d.L[q] = lens[q];
+    d.C[q] = cols[q];
+  }
+  return psx::fft::plan_create_with(n, kind, batch, &d, plan);
+}
+
 extern "C" int psx_fft_layout_check(uint64_t n, int kind, uint32_t batch, int npass, const int* lens, const int* cols) {
   PSX_REQUIRE(n >= 16 && (n % 2) == 0 && psx::fft::valid_complex_len(n / 2), "psx_fft_layout_check: bad length");
   PSX_REQUIRE(kind == PSX_FFT_R2C || kind == PSX_FFT_C2R, "psx_fft_layout_check: bad kind");

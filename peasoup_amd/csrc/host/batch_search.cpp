@@ -19,6 +19,7 @@
 #include "common.hpp"
 #include "peasoup/distiller.hpp"
 #include "search_common.hpp"
+#include "trace.hpp"
 
 using psx::batch::Engine;
 using psx::batch::EngineOptions;
@@ -320,12 +321,15 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
     // prologue is queued: building one later waits for the queued prologues
     eng->prepare((uint32_t)std::min<uint64_t>(eng->batch(), ntrials));
     if (ntrials > eng->batch()) eng->prepare((uint32_t)(ntrials % eng->batch()));
-    for (uint32_t d = 0; d < ndm; ++d) {
-      float* tim = d_tims + (uint64_t)d * N;
-      eng->prologue_async(input_of(d, ps), nreal, tim, &h_res[d], ps, zap.get());
-      PSX_CHECK_HIP(hipEventRecord(done_ev[d], ps));
-      tims[d] = tim;
-      idx[d] = (int)d;
+    {
+      psx::TraceRange dm_range("DM-Loop");  // the per-DM prologues, all queued here
+      for (uint32_t d = 0; d < ndm; ++d) {
+        float* tim = d_tims + (uint64_t)d * N;
+        eng->prologue_async(input_of(d, ps), nreal, tim, &h_res[d], ps, zap.get());
+        PSX_CHECK_HIP(hipEventRecord(done_ev[d], ps));
+        tims[d] = tim;
+        idx[d] = (int)d;
+      }
     }
     auto ready = [&](size_t d) {
       hip_check(hipEventSynchronize(done_ev[d]), "prologue event");
@@ -343,7 +347,10 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
     };
     auto t1 = clk::now();
     std::vector<std::vector<Candidate>> per_dm;
-    psx::search::run_trials(*eng, *args, tims, means, stds, dms, idx, acc_lists, per_dm, ready);
+    {
+      psx::TraceRange acc_range("Acceleration-Loop");  // every (DM, acceleration) trial, batched
+      psx::search::run_trials(*eng, *args, tims, means, stds, dms, idx, acc_lists, per_dm, ready);
+    }
     auto t2 = clk::now();
     std::vector<Candidate> flat;
     for (auto& v : per_dm) flat.insert(flat.end(), v.begin(), v.end());

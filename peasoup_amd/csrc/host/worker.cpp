@@ -14,6 +14,7 @@
 #include "peasoup/distiller.hpp"
 #include "peasoup/transforms.hpp"
 #include "search_common.hpp"
+#include "trace.hpp"
 
 namespace {
 
@@ -58,6 +59,7 @@ void run_worker(const psx_search_args& args, DispersionTrials<float>& trials, co
   float mean, std, rms;
   try {
     for (std::size_t ii = 0; ii < trials.get_count(); ++ii) {
+      psx::TraceRange dm_range("DM-Loop");
       auto t0 = clk::now();
       trials.get_idx(ii, tim, size);
       d_tim.copy_from_host(tim);
@@ -78,6 +80,7 @@ void run_worker(const psx_search_args& args, DispersionTrials<float>& trials, co
       auto t1 = clk::now();
       CandidateCollection accel_trial_cands;
       for (std::size_t jj = 0; jj < acc_list.size(); jj++) {
+        psx::TraceRange acc_range("Acceleration-Loop");
         resampler.resampleII(d_tim, d_tim_r, size, acc_list[jj]);
         r2cfft.execute(d_tim_r.get_data(), d_fseries.get_data());
         former.form_interpolated(d_fseries, pspec);

@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "kernels_dev.hpp"
 #include "reduce_async.hpp"
+#include "trace.hpp"
 
 using psx::as_stream;
 
@@ -682,6 +683,7 @@ extern "C" int psx_harmonic_sum(const float* P, float* const* folds, uint32_t nf
     fp.f[k] = folds[k];
   }
   if (!nb) return psx::ok();
+  psx::TraceRange range("Harmonic summing");
   const unsigned g = psx::grid_for(nb, kBlock);
   const hipStream_t hs = as_stream(s);
   switch (nfolds) {

@@ -84,6 +84,50 @@ def test_fft_r2c_vs_exact(psx, orc, n):
     assert err < 2e-6, (plan.describe(), err)
 
 
+def _tuned_decomps():
+    """Every FFT decomposition the shipped schedule table names."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "peasoup_amd", "tuned.json")
+    with open(path) as f:
+        table = json.load(f)
+    out = set()
+    for per_dev in table.values():
+        for n, e in per_dev.items():
+            if e.get("decomp"):
+                out.add((int(n), tuple(tuple(x) for x in e["decomp"])))
+    return sorted(out)
+
+
+# named decompositions beyond the defaults: strided radix-3 columns whose last
+# radix is at most 8 (the factored four-step twiddle), radix-3 rows, three
+# passes with a radix-3 first pass, and the 2^23 default and a three-pass
+# alternative named explicitly
+EXTRA_DECOMPS = [
+    (3 << 20, ((384, 16), (4096, 4))),
+    (3 << 20, ((1536, 8), (1024, 4))),
+    (3 << 20, ((1024, 16), (1536, 4))),
+    (3 << 21, ((96, 16), (128, 16), (256, 16))),
+    (1 << 23, ((1024, 16), (4096, 4))),
+    (1 << 23, ((128, 16), (128, 16), (256, 16))),
+]
+
+
+@pytest.mark.parametrize("n,decomp", _tuned_decomps() + EXTRA_DECOMPS)
+def test_fft_r2c_decomposition_vs_exact(psx, orc, n, decomp):
+    """Each schedule's FFT decomposition, built as a plain plan, is within
+    the same 2e-6 of the exact double DFT as the default plans."""
+    assert len(_tuned_decomps()) >= 3
+    x = RNG.standard_normal(n).astype(np.float32)
+    plan = psx.FFTPlan(n, psx.PSX_FFT_R2C, decomp=list(decomp))
+    desc = plan.describe()
+    assert "passes=" + "x".join("%d/C%d" % (L, c) for L, c in decomp) + " " in desc, desc
+    d_x, d_X = dev(psx, x), out(psx, n // 2 + 1, np.complex64)
+    plan.execute(d_x, d_X)
+    err = fft_err(d_X.numpy(), orc.fft_r2c_f64(x.astype(np.float64)))
+    assert err < 2e-6, (desc, err)
+
+
 @pytest.mark.parametrize("n", [16, 1024, 6144, 1 << 17, 3 * (1 << 17), 1 << 23])
 def test_fft_c2r_vs_exact(psx, orc, n):
     x = RNG.standard_normal(n).astype(np.float32)
