@@ -182,13 +182,16 @@ def serve_stream(P, W, nreq=64, rate=4.0, seed=0x5EA505, budget=200e9, batch=16,
     return res
 
 
-def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, schedules=None, log=log):
+def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, schedules=None, log=log, reps=3):
     """Segmented searches of uneven length: ndm DM trials, an equal mix of fft
     sizes 2^22, 3*2^21 and 2^23 at 76 us (each segment 90% of its fft size,
     so the series are zero padded after baseline removal), red noise and
     injections, accel -50..+50 m/s^2.  The three size classes are submitted
-    together to the server, whose per-size engines run concurrently."""
-    from peasoup_amd.serve import Server
+    together to the server, in both dispatch forms on the same inputs,
+    alternating reps times: per-size engines launching on their own streams,
+    and one graph launch per mixed-size batch (each size's batch graph a
+    child of it).  The faster form's best time is the line's value."""
+    from peasoup_amd.serve import DISPATCH_MIXED_GRAPH, DISPATCH_STREAMS, Server
     sizes = [1 << 22, 3 << 21, 1 << 23]
     per = [ndm // 3 + (1 if k < ndm % 3 else 0) for k in range(3)]
     reqs = []
@@ -201,20 +204,33 @@ def uneven_batched(P, W, ndm=512, seed=0x5EA503, batch=16, schedules=None, log=l
         reqs.append((w, W.generate_device(w)))
     P.synchronize()
     ntrials = sum(w.ndm * W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi).size for w, _ in reqs)
-    with Server(200e9, batch=batch, schedules=schedules) as srv:
-        # warm-up: engines for every size class
-        for w, d in reqs:
-            srv.wait(srv.submit(w.search_args(), d, w.dm_list()[:1], nsamps=w.n_real))
-        t0 = time.perf_counter()
-        ids = [srv.submit(w.search_args(), d, w.dm_list(), nsamps=w.n_real) for w, d in reqs]
-        sts = [srv.wait(i, timeout=600) for i in ids]
-        dt = time.perf_counter() - t0
-        ncand = sum(len(srv.result(i)) for i in ids)
-    if any(s["state"] != "done" for s in sts):
-        raise RuntimeError("uneven batch failed: %s" % sts)
+    forms = {"per-size streams": DISPATCH_STREAMS, "mixed-size graph": DISPATCH_MIXED_GRAPH}
+    best, ncand = {}, {}
+    for _ in range(reps):
+        for name, mode in forms.items():
+            with Server(200e9, batch=batch, schedules=schedules, dispatch=mode) as srv:
+                # warm-up: the whole workload once (engines, batch graphs and,
+                # for the mixed form, the mixed-batch graphs exist afterwards)
+                for i in [srv.submit(w.search_args(), d, w.dm_list(), nsamps=w.n_real) for w, d in reqs]:
+                    srv.wait(i, timeout=600)
+                    srv.result(i)
+                t0 = time.perf_counter()
+                ids = [srv.submit(w.search_args(), d, w.dm_list(), nsamps=w.n_real) for w, d in reqs]
+                sts = [srv.wait(i, timeout=600) for i in ids]
+                dt = time.perf_counter() - t0
+                if any(s["state"] != "done" for s in sts):
+                    raise RuntimeError("uneven batch failed: %s" % sts)
+                ncand[name] = sum(len(srv.result(i)) for i in ids)
+            best[name] = min(best.get(name, dt), dt)
+    if len(set(ncand.values())) != 1:
+        raise RuntimeError("uneven batch: dispatch forms disagree on the candidate count: %s" % ncand)
+    form = min(best, key=best.get)
+    dt = best[form]
     res = {"trials": int(ntrials), "dm_trials": ndm, "seconds": round(dt, 3), "trials_per_s": round(ntrials / dt, 1),
-           "candidates": ncand, "workload": "%d DM trials, equal mix of fft sizes 2^22, 3*2^21, 2^23 (segments "
-                                            "90%% of the fft size, zero padded), 76 us, accel -50..+50" % ndm}
+           "candidates": ncand[form], "form": form,
+           "forms_trials_per_s": {k: round(ntrials / v, 1) for k, v in best.items()},
+           "workload": "%d DM trials, equal mix of fft sizes 2^22, 3*2^21, 2^23 (segments "
+                       "90%% of the fft size, zero padded), 76 us, accel -50..+50" % ndm}
     log("uneven", res)
     return res
 

@@ -322,7 +322,13 @@ typedef struct psx_server_config {
   int32_t use_graph;          /* graph-launched batches */
   uint32_t nschedules;        /* per-length schedules (copied at create; may be 0) */
   const psx_schedule* schedules;
+  int32_t dispatch;           /* PSX_DISPATCH_STREAMS or PSX_DISPATCH_MIXED_GRAPH */
 } psx_server_config;
+/* How batches of different FFT lengths reach the device: one stream per
+ * length, each launching its own batch graphs (the lengths overlap as the
+ * streams allow), or one graph launch per mixed batch, whose child graphs
+ * are one batch of every length that has trials waiting. */
+enum { PSX_DISPATCH_STREAMS = 0, PSX_DISPATCH_MIXED_GRAPH = 1 };
 
 /* One beam: ndm dedispersed series of nsamps samples (host or device memory,
  * valid until the request completes) searched with args. */

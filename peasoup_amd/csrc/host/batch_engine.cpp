@@ -108,6 +108,7 @@ Engine::~Engine() {
   if (s) (void)hipStreamSynchronize(s);
   for (Slot& sl : slots) {
     for (auto& g : sl.graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : sl.defs) (void)hipGraphDestroy(g.second);
     (void)hipFree(sl.d_desc);
     (void)hipHostFree(sl.h_desc);
     (void)hipFree(sl.d_cands);
@@ -226,8 +227,10 @@ void Engine::launch(Slot& sl, uint32_t n, bool run) {
       }
       hip_ok(hipStreamEndCapture(s, &g), "end capture");
       hipGraphExec_t ge;
-      hip_ok(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
-      (void)hipGraphDestroy(g);
+      const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) (void)hipGraphDestroy(g);
+      hip_ok(e, "graph instantiate");
+      sl.defs.emplace(n, g);
       it = sl.graphs.emplace(n, ge).first;
     }
     if (!run) return;
@@ -246,6 +249,41 @@ void Engine::submit(const TrialWork* trials, uint32_t n) {
     collect_oldest(ready.back());
   }
   Slot& sl = slots[next_slot];
+  fill_slot(sl, trials, n);
+  enqueue(sl, n);
+  pending.push_back(next_slot);
+  next_slot ^= 1;
+}
+
+int Engine::stage(const TrialWork* trials, uint32_t n) {
+  if (!use_graph) throw std::runtime_error("stage: mixed dispatch needs launch graphs");
+  if (n == 0 || n > B) throw std::runtime_error("stage: 1..B trials");
+  if (pending.size() == 2) {
+    ready.emplace_back();
+    collect_oldest(ready.back());
+  }
+  Slot& sl = slots[next_slot];
+  fill_slot(sl, trials, n);
+  launch(sl, n, false);
+  return next_slot;
+}
+
+hipGraph_t Engine::graph_def(int slot, uint32_t n) const {
+  const auto it = slots[slot].defs.find(n);
+  if (it == slots[slot].defs.end()) throw std::runtime_error("graph_def: no graph for this batch");
+  return it->second;
+}
+
+void Engine::launched(int slot, hipStream_t ls) {
+  if (slot != next_slot) throw std::runtime_error("launched: slot was not the staged one");
+  hip_ok(hipEventRecord(slots[slot].done, ls), "event record");
+  pending.push_back(slot);
+  next_slot ^= 1;
+}
+
+// Trial descriptors of a batch into the slot's pinned buffer (the batch's
+// graph uploads them).
+void Engine::fill_slot(Slot& sl, const TrialWork* trials, uint32_t n) {
   sl.meta.assign(trials, trials + n);
   for (uint32_t t = 0; t < n; ++t) {
     const TrialWork& w = trials[t];
@@ -259,9 +297,6 @@ void Engine::submit(const TrialWork* trials, uint32_t n) {
     d.pad = 0;
     sl.h_desc[t] = d;
   }
-  enqueue(sl, n);
-  pending.push_back(next_slot);
-  next_slot ^= 1;
 }
 
 bool Engine::collect(std::vector<std::vector<Candidate>>& per_trial) {

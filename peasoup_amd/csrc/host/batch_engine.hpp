@@ -72,6 +72,15 @@ class Engine {
   // trials now, so that the first submits do not instantiate while other
   // streams' work is queued (instantiation waits for the device).
   void prepare(uint32_t n);
+  // Mixed-length dispatch (one graph launch carries a batch of several
+  // engines): stage() fills the next slot with up to B trials and makes sure
+  // its launch graph for n trials exists, without launching it, and returns
+  // the slot; the caller launches graph_def(slot, n), as a child of its own
+  // graph, on a stream of its choice and then calls launched(slot, stream),
+  // which records the slot's completion there.  Needs graphs enabled.
+  int stage(const TrialWork* trials, uint32_t n);
+  hipGraph_t graph_def(int slot, uint32_t n) const;
+  void launched(int slot, hipStream_t ls);
   // Waits for the oldest in-flight batch and returns its per-trial
   // harmonic-distilled candidate lists (trial order of the submit).
   bool collect(std::vector<std::vector<Candidate>>& per_trial);
@@ -114,7 +123,9 @@ class Engine {
     hipEvent_t done = nullptr;
     std::vector<TrialWork> meta;
     std::map<uint32_t, hipGraphExec_t> graphs;
+    std::map<uint32_t, hipGraph_t> defs;  // the captured graphs, for mixed dispatch
   };
+  void fill_slot(Slot& sl, const TrialWork* trials, uint32_t n);
   void enqueue(Slot& sl, uint32_t n);
   // enqueue's body; run = false only builds the slot's graph for n
   void launch(Slot& sl, uint32_t n, bool run);

@@ -10,8 +10,11 @@
 //             join the trial queue of the engine for its shape;
 //   dispatch  keeps two batches in flight per engine, filling each batch from
 //             the head of that engine's queue - trials of different requests
-//             share a batch, and engines of different shapes run on their own
-//             streams so their work overlaps on the device;
+//             share a batch.  Engines of different shapes either launch on
+//             their own streams, so their work overlaps on the device
+//             (PSX_DISPATCH_STREAMS), or take part in one graph launch per
+//             mixed batch whose child graphs are one batch of every engine
+//             with trials waiting (PSX_DISPATCH_MIXED_GRAPH);
 //   collect   routes finished trials back to their requests; a request whose
 //             trials are all done gets the Worker's per-DM
 //             AccelerationDistiller, releases its memory and is completed.
@@ -72,6 +75,7 @@ struct TrialRef {
 struct EngineState {
   std::unique_ptr<Engine> eng;
   hipStream_t stream = nullptr;
+  hipEvent_t mark = nullptr;  // the engine stream's work so far (mixed dispatch waits on it)
   uint64_t bytes = 0;
   std::deque<TrialRef> queue;        // trials waiting for a batch
   std::deque<std::vector<TrialRef>> inflight;  // trials of each submitted batch
@@ -119,6 +123,10 @@ struct psx_server {
   std::vector<Request*> active;
   std::map<uint64_t, std::unique_ptr<Request>> all;
   std::map<EngineKey, std::unique_ptr<EngineState>> engines;
+  // mixed dispatch: one stream, and the instantiated parent graph of every
+  // combination of (engine, slot, batch size) launched so far
+  hipStream_t mixed_stream = nullptr;
+  std::map<std::vector<std::tuple<EngineState*, int, uint32_t>>, hipGraphExec_t> mixed_graphs;
   uint64_t used = 0;
   uint64_t completed = 0, failed = 0, trials_done = 0;
   int device = 0;
@@ -133,6 +141,8 @@ struct psx_server {
   void fail_request(Request* r, int code, const std::string& msg);
   EngineState* engine_for(const psx_search_args& a, bool create, uint64_t* need);
   void evict_idle(uint64_t need);
+  bool dispatch_mixed();
+  void drop_mixed_graphs(const EngineState* es);
 };
 
 namespace {
@@ -159,6 +169,10 @@ EngineState* psx_server::engine_for(const psx_search_args& a, bool create, uint6
   std::unique_ptr<EngineState> es(new EngineState);
   if (hipStreamCreateWithFlags(&es->stream, hipStreamNonBlocking) != hipSuccess)
     throw std::runtime_error("stream creation failed");
+  if (hipEventCreateWithFlags(&es->mark, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(es->stream);
+    throw std::runtime_error("event creation failed");
+  }
   EngineOptions eo;
   eo.batch = batch_for(a.size);
   eo.use_graph = cfg.use_graph != 0;
@@ -189,10 +203,87 @@ void psx_server::evict_idle(uint64_t need) {
     }
     if (victim == engines.end()) return;
     used -= victim->second->bytes;
+    drop_mixed_graphs(victim->second.get());
     victim->second->eng.reset();
     (void)hipStreamDestroy(victim->second->stream);
+    (void)hipEventDestroy(victim->second->mark);
     engines.erase(victim);
   }
+}
+
+void psx_server::drop_mixed_graphs(const EngineState* es) {
+  for (auto it = mixed_graphs.begin(); it != mixed_graphs.end();) {
+    bool uses = false;
+    for (const auto& part : it->first) uses |= std::get<0>(part) == es;
+    if (uses) {
+      (void)hipGraphExecDestroy(it->second);
+      it = mixed_graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+// One mixed batch: every engine with trials waiting and a free slot stages a
+// batch, and one launch of a parent graph (the staged batches' graphs as
+// independent children) runs them all on the mixed stream, after each
+// engine's own stream work (its requests' prologues).
+bool psx_server::dispatch_mixed() {
+  std::vector<EngineState*> part;
+  for (auto& kv : engines) {
+    EngineState* es = kv.second.get();
+    if (!es->queue.empty() && es->eng->in_flight() < 2) part.push_back(es);
+  }
+  if (part.empty()) return false;
+  if (!mixed_stream && hipStreamCreateWithFlags(&mixed_stream, hipStreamNonBlocking) != hipSuccess)
+    throw std::runtime_error("stream creation failed");
+  std::vector<std::tuple<EngineState*, int, uint32_t>> key;
+  std::vector<TrialWork> work;
+  for (EngineState* es : part) {
+    Engine& eng = *es->eng;
+    const uint32_t n = (uint32_t)std::min<size_t>(eng.batch(), es->queue.size());
+    work.clear();
+    std::vector<TrialRef> refs(es->queue.begin(), es->queue.begin() + n);
+    es->queue.erase(es->queue.begin(), es->queue.begin() + n);
+    for (const TrialRef& t : refs) {
+      Request* r = t.req;
+      work.push_back(TrialWork{r->d_tims + (uint64_t)t.dm * r->spec.args.size, r->acc_lists[t.dm][t.acc],
+                               r->means[t.dm], r->stds[t.dm], r->dm_list[t.dm], (int)t.dm});
+    }
+    const int slot = eng.stage(work.data(), n);
+    es->inflight.push_back(std::move(refs));
+    es->last_used = clk::now();
+    key.emplace_back(es, slot, n);
+  }
+  auto it = mixed_graphs.find(key);
+  if (it == mixed_graphs.end()) {
+    if (mixed_graphs.size() >= 256) {
+      for (auto& g : mixed_graphs) (void)hipGraphExecDestroy(g.second);
+      mixed_graphs.clear();
+    }
+    hipGraph_t pg;
+    if (hipGraphCreate(&pg, 0) != hipSuccess) throw std::runtime_error("hipGraphCreate failed");
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipSuccess;
+    for (const auto& k : key) {
+      hipGraphNode_t node;
+      e = hipGraphAddChildGraphNode(&node, pg, nullptr, 0,
+                                    std::get<0>(k)->eng->graph_def(std::get<1>(k), std::get<2>(k)));
+      if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&ge, pg, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(pg);
+    if (e != hipSuccess) throw std::runtime_error(std::string("mixed batch graph: ") + hipGetErrorString(e));
+    it = mixed_graphs.emplace(key, ge).first;
+  }
+  for (EngineState* es : part) {
+    if (hipEventRecord(es->mark, es->stream) != hipSuccess ||
+        hipStreamWaitEvent(mixed_stream, es->mark, 0) != hipSuccess)
+      throw std::runtime_error("mixed batch: stream ordering failed");
+  }
+  if (hipGraphLaunch(it->second, mixed_stream) != hipSuccess) throw std::runtime_error("mixed batch: launch failed");
+  for (const auto& k : key) std::get<0>(k)->eng->launched(std::get<1>(k), mixed_stream);
+  return true;
 }
 
 void psx_server::fail_request(Request* r, int code, const std::string& msg) {
@@ -315,11 +406,13 @@ void psx_server::loop() {
         progressed = true;
       }
       try {
+        if (cfg.dispatch == PSX_DISPATCH_MIXED_GRAPH)
+          while (dispatch_mixed()) progressed = true;
         // dispatch + collect per engine
         for (auto& kv : engines) {
           EngineState* es = kv.second.get();
           Engine& eng = *es->eng;
-          while (!es->queue.empty() && eng.in_flight() < 2) {
+          while (cfg.dispatch == PSX_DISPATCH_STREAMS && !es->queue.empty() && eng.in_flight() < 2) {
             const uint32_t n = (uint32_t)std::min<size_t>(eng.batch(), es->queue.size());
             work.clear();
             std::vector<TrialRef> refs(es->queue.begin(), es->queue.begin() + n);
@@ -366,6 +459,8 @@ extern "C" int psx_server_create(const psx_server_config* cfg, psx_server** out)
   PSX_REQUIRE(cfg && out, "psx_server_create: null argument");
   PSX_REQUIRE(cfg->hbm_budget_bytes > 0, "psx_server_create: zero budget");
   PSX_REQUIRE(!cfg->nschedules || cfg->schedules, "psx_server_create: null schedules");
+  PSX_REQUIRE(cfg->dispatch == PSX_DISPATCH_STREAMS || (cfg->dispatch == PSX_DISPATCH_MIXED_GRAPH && cfg->use_graph),
+              "psx_server_create: dispatch must be PSX_DISPATCH_STREAMS, or PSX_DISPATCH_MIXED_GRAPH with graphs");
   psx_server* s = new psx_server;
   s->cfg = *cfg;
   if (!s->cfg.batch) s->cfg.batch = 16;
@@ -500,10 +595,14 @@ extern "C" int psx_server_destroy(psx_server* s) {
     s->cv.notify_all();
   }
   if (s->sched.joinable()) s->sched.join();
+  if (s->mixed_stream) (void)hipStreamSynchronize(s->mixed_stream);
+  for (auto& g : s->mixed_graphs) (void)hipGraphExecDestroy(g.second);
   for (auto& kv : s->engines) {
     kv.second->eng.reset();
     (void)hipStreamDestroy(kv.second->stream);
+    (void)hipEventDestroy(kv.second->mark);
   }
+  if (s->mixed_stream) (void)hipStreamDestroy(s->mixed_stream);
   for (auto& kv : s->all)
     if (kv.second->d_tims) (void)hipFree(kv.second->d_tims);
   delete s;

@@ -19,7 +19,11 @@ class Schedule(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("batch", C.c_uint32), ("max_active", C.c_uint32),
-                ("use_graph", C.c_int32), ("nschedules", C.c_uint32), ("schedules", C.POINTER(Schedule))]
+                ("use_graph", C.c_int32), ("nschedules", C.c_uint32), ("schedules", C.POINTER(Schedule)),
+                ("dispatch", C.c_int32)]
+
+
+DISPATCH_STREAMS, DISPATCH_MIXED_GRAPH = 0, 1
 
 
 class Request(C.Structure):
@@ -73,11 +77,14 @@ def schedules_from(table):
 
 
 class Server:
-    def __init__(self, budget_bytes, batch=16, max_active=0, use_graph=True, schedules=None):
-        """schedules: optional {fft size: {"batch", "decomp"}} (the tuned table)."""
+    def __init__(self, budget_bytes, batch=16, max_active=0, use_graph=True, schedules=None,
+                 dispatch=DISPATCH_STREAMS):
+        """schedules: optional {fft size: {"batch", "decomp"}} (the tuned table).
+        dispatch: DISPATCH_STREAMS (one stream per FFT length) or
+        DISPATCH_MIXED_GRAPH (one graph launch per mixed-length batch)."""
         arr, n = schedules_from(schedules or {})
         self.cfg = ServerConfig(int(budget_bytes), batch, max_active, 1 if use_graph else 0, n,
-                                C.cast(arr, C.POINTER(Schedule)))
+                                C.cast(arr, C.POINTER(Schedule)), dispatch)
         h = C.c_void_p()
         P.check(P.lib().psx_server_create(C.byref(self.cfg), C.byref(h)))
         self.h = h.value

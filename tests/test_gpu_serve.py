@@ -188,3 +188,25 @@ def test_serving_sizes_alone_and_in_the_mix(psx):
         c = direct[k][direct[k]["dm_idx"] == inj.trial]
         r = c["freq"] * inj.period
         assert np.any((c["snr"] > 9) & (np.abs(r - np.round(r)) < 0.01)), (w.name, c[:5])
+
+
+def test_mixed_graph_dispatch_same_results(psx, reqs):
+    """One graph launch per mixed-length batch (each length's batch graph a
+    child of it) returns every request's candidates exactly as the
+    per-length streams do, and as the request alone."""
+    from peasoup_amd.serve import DISPATCH_MIXED_GRAPH, DISPATCH_STREAMS, Server
+    results = {}
+    for mode in (DISPATCH_STREAMS, DISPATCH_MIXED_GRAPH):
+        with Server(64 << 30, batch=8, dispatch=mode) as srv:
+            ids = [srv.submit(w.search_args(), s, w.dm_list()) for w, s in reqs]
+            sts = [srv.wait(i) for i in ids]
+            assert all(st["state"] == "done" for st in sts), (mode, sts)
+            results[mode] = [srv.result(i, forest=True) for i in ids]
+            assert srv.stats()["engines"] >= 3
+    for k, (w, _) in enumerate(reqs):
+        a, b = results[DISPATCH_STREAMS][k], results[DISPATCH_MIXED_GRAPH][k]
+        assert same(a, b) and len(a) == len(b), w.name
+    assert sum(len(r) for r in results[DISPATCH_MIXED_GRAPH]) > 0
+    import peasoup_amd as P
+    with pytest.raises(P.PsxError):
+        Server(64 << 30, batch=8, use_graph=False, dispatch=DISPATCH_MIXED_GRAPH)
