@@ -12,7 +12,9 @@ threshold (tests/candmatch.py).
 
 Lengths: 2^23 at 76 us with red noise (the 64 x 2^23 workload's shape),
 3*2^21 and 2^22 with 90% real samples (zero-padded segments of the uneven
-workload), and 2^26 at 64 us (the long-segment workload).  Each test takes a
+workload), 2^26 at 64 us (the long-segment workload), 2^20 and 2^24 at
+64 us (the ends of the serving workload's range), and 2^22 with five
+harmonic folds at the example run's settings.  Each test takes a
 subset of the acceleration plan that includes the injected pulsar's
 acceleration, so the pulsar and its harmonics are found and the noise
 candidates around the threshold are compared as well.
@@ -105,3 +107,30 @@ def test_parity_2pow26_long_segment(psx, orc):
     w.injections = [W.Injection(0, 0.00427, float(accs_all[k]), 25.0, 0.1)]
     accs = accs_all[k - 1:k + 2]
     _check(psx, orc, w, accs, 1 / 0.00427)
+
+
+@pytest.mark.parametrize("n,acc_range", [(1 << 20, 50.0), (1 << 24, 200.0)])
+def test_parity_serving_lengths(psx, orc, n, acc_range):
+    """Serving-workload lengths at the ends of its range: 2^20 and 2^24
+    samples at 64 us with red noise, accel +-50 or +-200 m/s^2."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("serve%d" % n, 2, n, 64e-6, 0x5EA5C5 + n % 89, 5.0, -acc_range, acc_range)
+    accs_all = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
+    inj_acc = float(accs_all[(2 * accs_all.size) // 3])
+    w.injections = [W.Injection(1, 0.0093, inj_acc, 20.0, 0.6)]
+    accs = _accel_subset(accs_all, max(1, accs_all.size // 6), inj_acc)
+    _check(psx, orc, w, accs, 1 / 0.0093)
+
+
+def test_parity_five_folds_example_run_settings(psx, orc):
+    """The example run's search settings (5 harmonic folds, S/N 6, acc_tol
+    1.11, example_output/overview.xml:40-62) at 2^22 samples."""
+    import peasoup_amd.workloads as W
+    w = W.Workload("five", 2, 1 << 22, 64e-6, 0x5EA5C6, 5.0, -150.0, 150.0, nharmonics=5, min_snr=6.0)
+    accs_all = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi, tol=1.11)
+    inj_acc = float(accs_all[accs_all.size // 4])
+    w.injections = [W.Injection(0, 0.0031, inj_acc, 18.0, 0.2)]
+    accs = _accel_subset(accs_all, max(1, accs_all.size // 5), inj_acc)
+    got, ref = _check(psx, orc, w, accs, 1 / 0.0031)
+    print("harmonic numbers of the matched candidates:", np.bincount(ref["nh"], minlength=6), flush=True)
+    assert set(got["nh"]) <= set(range(6))
