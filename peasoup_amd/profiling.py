@@ -18,10 +18,15 @@ Two inputs:
   CU issues at most one wave64 VALU instruction per cycle over its four
   SIMDs; GRBM_GUI_ACTIVE counts the busy cycles of every XCD).
 
+* counter runs of per-unit busy/stall cycle counters (``TA_TA_BUSY_sum``,
+  ``TD_TD_BUSY_sum``, ``TCP_PENDING_STALL_CYCLES_sum``, ...) -> each as a share
+  of the kernel's cycles on every CU.
+
 Usage:
   python -m peasoup_amd.profiling stats <dir>            # kernel stats table
   python -m peasoup_amd.profiling traffic <fetch_dir> <write_dir> --n 8388608 --trials 16 [--out traffic.json]
   python -m peasoup_amd.profiling issue <dir> [--cus 256 --xcds 8]
+  python -m peasoup_amd.profiling busy <dir> [<dir> ...] [--cus 256 --xcds 8]
 """
 import argparse
 import csv
@@ -199,6 +204,46 @@ def issue(d, cus=256, xcds=8):
     return per
 
 
+def busy(dirs, cus=256, xcds=8):
+    """Per batched stage: each busy/stall counter of the runs in dirs
+    (per-unit cycle counts summed over the device, e.g. TA_TA_BUSY_sum,
+    TD_TD_BUSY_sum) as a share of the kernel's cycles on every CU
+    (GRBM_GUI_ACTIVE / xcds per run), and a *_LATENCY counter over its
+    request counter as cycles per request, averaged over the stage's batched
+    launches."""
+    per, fft_rank = {}, {}
+    for d in dirs:
+        disp = _dispatches(d)
+        first = min((i for i, (k, _, _) in disp.items() if stage_of(k) == "fft_pass0_resample"), default=0)
+        groups = {}
+        for i, (k, g, c) in disp.items():
+            if i >= first and stage_of(k) is not None:
+                groups.setdefault(k, []).append((g, i, c))
+        ranks = []
+        for k, vals in groups.items():
+            gmax = max(g for g, _, _ in vals)
+            big = [(i, c) for g, i, c in vals if g == gmax]
+            avg = {n: sum(c[n] for _, c in big) / len(big) for n in big[0][1]}
+            cycles = avg.pop("GRBM_GUI_ACTIVE") / xcds
+            shares = {}
+            for n, v in avg.items():
+                base = n.replace("_sum", "")
+                if base.endswith("_LATENCY"):
+                    req = avg.get(n.replace("_LATENCY", ""))
+                    if req:
+                        shares[base + "_PER_REQ"] = v / req  # cycles per request
+                elif base.endswith(("BUSY", "CYCLES", "STALL")):
+                    shares[base] = v / (cus * cycles)
+            st = stage_of(k)
+            if st == "fft_pass":
+                ranks.append((min(i for i, _ in big), k, shares))
+            else:
+                per.setdefault(st, {"kernel": k}).update(shares)
+        for q, (_, k, shares) in enumerate(sorted(ranks, key=lambda x: x[0]), start=1):
+            per.setdefault("fft_pass%d" % q, {"kernel": k}).update(shares)
+    return per
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -215,7 +260,20 @@ def main(argv=None):
     s3.add_argument("dir")
     s3.add_argument("--cus", type=int, default=256)
     s3.add_argument("--xcds", type=int, default=8)
+    s4 = sub.add_parser("busy")
+    s4.add_argument("dirs", nargs="+")
+    s4.add_argument("--cus", type=int, default=256)
+    s4.add_argument("--xcds", type=int, default=8)
     a = ap.parse_args(argv)
+    if a.cmd == "busy":
+        per = busy(a.dirs, a.cus, a.xcds)
+        names = sorted({n for e in per.values() for n in e if n != "kernel"})
+        print("%-20s %-36s" % ("stage", "kernel") + "".join(" %30s" % n for n in names))
+        for st, e in sorted(per.items()):
+            print("%-20s %-36s" % (st, e["kernel"][:36]) +
+                  "".join(" %30s" % (("%.0f" if n.endswith("_PER_REQ") else "%.3f") % e[n] if n in e else "-")
+                    for n in names))
+        return
     if a.cmd == "issue":
         print("%-20s %-36s %9s %10s %9s %8s %9s %10s %9s" % ("stage", "kernel", "waves", "cycles", "VALU/wave",
                                                            "LDS/wave", "VMEM/wave", "VALU issue", "LDS issue"))
