@@ -29,3 +29,26 @@ def test_stage_bytes_model():
     assert three[:3] == [4 * n + 8 * m, 16 * m, 16 * m] and three[3:] == two[2:]
     # the fused trial moves 24n bytes with two FFT passes (about 28n with three)
     assert abs(sum(two) - 24 * n) <= 16
+
+
+def test_cpu_baseline_threads_and_model():
+    """The CPU baseline uses every CPU this process may run on, up to the
+    16-CPU share of a one-GPU host, and names the CPU model."""
+    sys.path.insert(0, ROOT)
+    import bench
+    n = bench.cpu_threads()
+    assert 1 <= n <= 16
+    assert n == min(16, len(os.sched_getaffinity(0)))
+    assert isinstance(bench.cpu_model(), str) and bench.cpu_model()
+
+
+def test_serving_stream_shape():
+    """The default serving stream is the serving workload's: 64 requests of
+    16..256 DM trials."""
+    sys.path.insert(0, ROOT)
+    import inspect
+    import bench
+    sig = inspect.signature(bench.serve_stream)
+    assert sig.parameters["nreq"].default == 64
+    assert sig.parameters["dms"].default == (16, 256)
+    assert sig.parameters["rate"].default == 4.0
