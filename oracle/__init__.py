@@ -31,14 +31,17 @@ class SearchParams(C.Structure):
                 ("max_freq", C.c_float), ("nharmonics", C.c_int32), ("freq_tol", C.c_float),
                 ("max_harm", C.c_float), ("min_gap", C.c_int32),
                 ("boundary_5_freq", C.c_float), ("boundary_25_freq", C.c_float), ("nbirdies", C.c_int32),
-                ("birdie_freq", C.c_void_p), ("birdie_width", C.c_void_p)]
+                ("birdie_freq", C.c_void_p), ("birdie_width", C.c_void_p), ("fast_fft", C.c_int32)]
 
 
 def make_params(size, tsamp, dm=0.0, dm_idx=0, min_snr=7.0, min_freq=0.1, max_freq=1100.0,
                 nharmonics=4, freq_tol=1e-4, max_harm=16.0, min_gap=30,
-                boundary_5_freq=0.05, boundary_25_freq=0.5, birdies=None):
+                boundary_5_freq=0.05, boundary_25_freq=0.5, birdies=None, fast_fft=False):
+    """fast_fft: the float four-step FFT (CPU baseline timing) instead of the
+    exact double DFT the checks use."""
     p = SearchParams(size, tsamp, dm, dm_idx, min_snr, min_freq, max_freq, nharmonics,
-                     freq_tol, max_harm, min_gap, boundary_5_freq, boundary_25_freq, 0, None, None)
+                     freq_tol, max_harm, min_gap, boundary_5_freq, boundary_25_freq, 0, None, None,
+                     1 if fast_fft else 0)
     if birdies is not None and len(birdies):
         b = np.asarray(birdies, np.float32).reshape(-1, 2)
         p._birdies = (np.ascontiguousarray(b[:, 0]), np.ascontiguousarray(b[:, 1]))
@@ -73,6 +76,8 @@ def lib():
             "orc_fft_r2c_f64": (C.c_int, [f64p, f64p, u64]),
             "orc_fft_c2r": (C.c_int, [f32p, f32p, u64]),
             "orc_fft_c2c_f64": (C.c_int, [f64p, u64, i32]),
+            "orc_fft_r2c_fast": (C.c_int, [f32p, f32p, u64]),
+            "orc_fft_c2r_fast": (C.c_int, [f32p, f32p, u64]),
             "orc_form_amplitude": (None, [f32p, f32p, u64]),
             "orc_form_interbin": (None, [f32p, f32p, u64]),
             "orc_median_scrunch5": (u64, [f32p, u64, f32p]),
@@ -159,6 +164,23 @@ def fft_r2c(x):
     if lib().orc_fft_r2c(x, out, x.size):
         raise ValueError("unsupported FFT size %d" % x.size)
     return out.view(np.complex64)
+
+
+def fft_r2c_fast(x):
+    """The CPU baseline's float four-step R2C (unnormalised), n/2+1 bins."""
+    x = _f32(x)
+    out = np.empty(2 * (x.size // 2 + 1), np.float32)
+    if lib().orc_fft_r2c_fast(x, out, x.size):
+        raise ValueError("unsupported FFT size %d" % x.size)
+    return out.view(np.complex64)
+
+
+def fft_c2r_fast(X, n):
+    X = np.ascontiguousarray(X, dtype=np.complex64).view(np.float32)
+    out = np.empty(n, np.float32)
+    if lib().orc_fft_c2r_fast(X, out, n):
+        raise ValueError("unsupported FFT size %d" % n)
+    return out
 
 
 def fft_r2c_f64(x):

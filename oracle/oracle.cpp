@@ -19,11 +19,15 @@
  *
  * The FFT (cuFFT in the reference, ffter.hpp:54-110) is restated as an exact
  * double-precision DFT (mixed radix 4/2/3 Stockham), whose outputs are rounded
- * to float where the reference hands float spectra on.
+ * to float where the reference hands float spectra on.  For the timed CPU
+ * baseline only (orc_search_params.fast_fft), a float four-step FFT
+ * (fastfft.cpp) takes its place, as FFTW3f would on the reference's host.
  *
  * Build: oracle/Makefile (g++ -O3 -march=x86-64-v3 -ffp-contract=off).
  */
 #include "oracle.h"
+
+#include "fastfft.h"
 
 #include <algorithm>
 #include <atomic>
@@ -956,6 +960,7 @@ namespace {
 struct TrialScratch {
   std::vector<float> tim_r, X, P, folds;
   std::vector<cd> z, work, Xd;
+  std::vector<orc_fast::cf> fz, fzo, fwork;  // float FFT buffers (fast_fft)
 };
 
 /* pipeline_multi.cu:130-131: float tobs = size*tsamp; float bin_width = 1.0/tobs */
@@ -986,7 +991,13 @@ int run_trial(const float* tim, const orc_search_params* p, float acc, float mea
   const unsigned F = (unsigned)std::min(std::max(p->nharmonics, 0), 5);
   s.folds.resize((size_t)F * nb);
   orc_resample2(tim, s.tim_r.data(), N, acc, p->tsamp);
-  rfft_float(rp, s.tim_r.data(), s.X.data(), s);
+  if (p->fast_fft) {
+    const orc_fast::RealPlan* fp = orc_fast::real_plan(N);
+    if (!fp) return -1;
+    orc_fast::rfft(*fp, s.tim_r.data(), s.X.data(), s.fz, s.fzo, s.fwork);
+  } else {
+    rfft_float(rp, s.tim_r.data(), s.X.data(), s);
+  }
   orc_form_interbin(s.X.data(), s.P.data(), nb);
   /* stats::normalise(pspec, mean*size, std*size, size/2+1) */
   orc_normalise(s.P.data(), mean * (float)(unsigned)N, std_ * (float)(unsigned)N, nb);
@@ -1032,7 +1043,12 @@ extern "C" int orc_prologue(const float* series, uint64_t n_real, const orc_sear
     for (uint64_t i = n_real; i < N; ++i) tim[i] = 0.0f;
   TrialScratch s;
   std::vector<float> X(2 * nb), A(nb), med(nb), P(nb);
-  rfft_float(rp, tim.data(), X.data(), s);
+  const orc_fast::RealPlan* fp = p->fast_fft ? orc_fast::real_plan(N) : nullptr;
+  if (p->fast_fft && !fp) return -1;
+  if (fp)
+    orc_fast::rfft(*fp, tim.data(), X.data(), s.fz, s.fzo, s.fwork);
+  else
+    rfft_float(rp, tim.data(), X.data(), s);
   orc_form_amplitude(X.data(), A.data(), nb);
   orc_calculate_median(A.data(), nb, (double)bin_width_of(p), p->boundary_5_freq, p->boundary_25_freq, med.data());
   orc_divide_c_by_f(X.data(), med.data(), nb);
@@ -1040,8 +1056,28 @@ extern "C" int orc_prologue(const float* series, uint64_t n_real, const orc_sear
   if (p->nbirdies > 0) orc_zap_birdies(X.data(), nb, bin_width_of(p), p->birdie_freq, p->birdie_width, p->nbirdies);
   orc_form_interbin(X.data(), P.data(), nb);
   orc_stats(P.data(), nb, 0, mean, rms, std_);
+  if (fp) {
+    orc_fast::irfft(*fp, X.data(), tim_out, s.fz, s.fzo, s.fwork);
+    return 0;
+  }
   std::vector<cd> z, work;
   irfft(rp, X.data(), tim_out, z, work);
+  return 0;
+}
+
+extern "C" int orc_fft_r2c_fast(const float* in, float* out, uint64_t n) {
+  const orc_fast::RealPlan* fp = orc_fast::real_plan(n);
+  if (!fp) return -1;
+  std::vector<orc_fast::cf> z, zo, work;
+  orc_fast::rfft(*fp, in, out, z, zo, work);
+  return 0;
+}
+
+extern "C" int orc_fft_c2r_fast(const float* in, float* out, uint64_t n) {
+  const orc_fast::RealPlan* fp = orc_fast::real_plan(n);
+  if (!fp) return -1;
+  std::vector<orc_fast::cf> z, zo, work;
+  orc_fast::irfft(*fp, in, out, z, zo, work);
   return 0;
 }
 

@@ -74,6 +74,8 @@ typedef struct {
   int32_t nbirdies;       /* zap list (Zapper): 0 = none */
   const float* birdie_freq;
   const float* birdie_width;
+  int32_t fast_fft;       /* 0: exact double DFT (the checker); 1: the float
+                             four-step FFT of fastfft.cpp (CPU baseline timing) */
 } orc_search_params;
 
 /* include/utils/utils.hpp:143-233 */
@@ -95,6 +97,9 @@ int orc_fft_r2c(const float* in, float* out, uint64_t n);
 int orc_fft_r2c_f64(const double* in, double* out, uint64_t n);
 int orc_fft_c2r(const float* in, float* out, uint64_t n);
 int orc_fft_c2c_f64(double* data, uint64_t m, int sign);
+/* the float four-step FFT of the CPU baseline (fastfft.cpp) */
+int orc_fft_r2c_fast(const float* in, float* out, uint64_t n);
+int orc_fft_c2r_fast(const float* in, float* out, uint64_t n);
 
 /* src/kernels.cu:216-234 (way 0) and :238-259 (way 1) */
 void orc_form_amplitude(const float* X, float* A, uint64_t nb);

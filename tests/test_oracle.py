@@ -347,3 +347,44 @@ def test_oracle_trial_finds_injected_tone():
     assert cands.size > 0
     best = cands[np.argmax(cands["snr"])]
     assert abs(best["freq"] - 37.0) < 2 * (1 / (n * float(ts)))
+
+
+# ------------------------------------------- float FFT of the CPU baseline
+@pytest.mark.parametrize("n", [16, 1024, 8192, 6144, 3 << 13, 1 << 17, 3 << 17, 1 << 20, 1 << 23])
+def test_fast_fft_against_exact(n):
+    """The CPU baseline's float four-step FFT agrees with the exact double
+    DFT to single-precision rounding (2e-6 of the RMS bin, the bound the GPU
+    FFT is held to), and its C2R inverts it (times n)."""
+    x = RNG.standard_normal(n).astype(np.float32)
+    X = O.fft_r2c_fast(x)
+    ref = O.fft_r2c_f64(x.astype(np.float64))
+    scale = np.sqrt(np.mean(np.abs(ref) ** 2))
+    assert np.max(np.abs(X.astype(np.complex128) - ref)) / scale < 2e-6
+    y = O.fft_c2r_fast(X, n)
+    assert np.max(np.abs(y / n - x)) / np.sqrt(np.mean(x.astype(np.float64) ** 2)) < 4e-6
+    # impulse: a flat spectrum; the closed form of a tone
+    d = np.zeros(n, np.float32)
+    d[0] = 1.0
+    assert np.allclose(O.fft_r2c_fast(d), 1.0, atol=1e-6)
+    k = n // 8
+    tone = np.cos(2 * np.pi * k * np.arange(n) / n).astype(np.float32)
+    T = O.fft_r2c_fast(tone)
+    assert abs(T[k] - n / 2) < 1e-3 * n and np.max(np.abs(np.delete(T, k))) < 1e-4 * n
+
+
+def test_fast_fft_search_matches_exact():
+    """A search with the float FFT finds the exact-FFT search's candidates
+    (2^17 samples, an injected 5.3 ms pulsar, red noise)."""
+    import sys
+    sys.path.insert(0, HERE)
+    from candmatch import compare
+    n = 1 << 17
+    t = np.arange(n) * 64e-6
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(n) + 0.5 * np.exp(-0.5 * ((t % 5.3e-3 - 2e-3) / 1.2e-4) ** 2)
+    x = (x + 0.02 * np.cumsum(rng.standard_normal(n))).astype(np.float32)
+    accs = np.array([-20.0, 0.0, 20.0], np.float32)
+    exact, _ = O.search_dm(x, O.make_params(n, f32(64e-6)), accs, nthreads=2)
+    fast, _ = O.search_dm(x, O.make_params(n, f32(64e-6), fast_fft=True), accs, nthreads=2)
+    assert len(exact) > 0
+    assert not compare(fast, exact, 7.0)
