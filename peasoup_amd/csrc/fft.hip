@@ -203,10 +203,11 @@ __device__ inline void run_stage(float2 (&v)[points_per_thread(L)], float2 (&tw)
         }
         if (a.out_conj) val.y = -val.y;
         // column passes write whole lines (C*8 bytes per row, 128 for C=16)
-        // and their output is read once by the next pass: streaming stores
-        // (measured 3% faster end to end at 2^23; the 32-byte runs of the
-        // rows-to-columns pass are 2x slower that way)
-        if constexpr (MODE == PM_COLS_TW)
+        // and contiguous-row passes whole rows, each read once by the next
+        // pass: streaming stores (measured 3% faster end to end at 2^23 for
+        // the column passes and 1.5% more for the row passes; the 32-byte
+        // runs of the rows-to-columns pass are 2x slower that way)
+        if constexpr (MODE == PM_COLS_TW || MODE == PM_CONTIG)
           stg2_nt(out, o0 + (uint32_t)(r * (L / R)) * oel, val);
         else
           stg2(out, o0 + (uint32_t)(r * (L / R)) * oel, val);
@@ -267,6 +268,8 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
       }
     }
   } else {
+    // every input element is read once: streaming loads (0.4% end to end
+    // at 2^23 on top of the streaming row stores)
     const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)blockIdx.z * a.in_tr;
     auto load = [&](unsigned cbase, float2(&dst)[P]) {
 #pragma unroll
@@ -276,7 +279,7 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
         const uint32_t o0 = (cbase + (uint32_t)c) * icol + (uint32_t)j * iel;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          float2 x = ldg2(in, o0 + (uint32_t)(r * (L / R0)) * iel);
+          float2 x = ldg2_nt(in, o0 + (uint32_t)(r * (L / R0)) * iel);
           if (a.in_conj) x.y = -x.y;
           dst[b * R0 + r] = x;
         }

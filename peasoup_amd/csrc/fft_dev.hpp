@@ -137,6 +137,12 @@ __device__ inline void stg2(float2* base, uint32_t i, float2 v) {
   w.y = v.y;
   *(PSX_GLOBAL psx_f2v*)((PSX_GLOBAL char*)base + (uint64_t)(i * 8u)) = w;
 }
+// Streaming (non-temporal) load: the pass reads each element once.
+__device__ inline float2 ldg2_nt(const float2* base, uint32_t i) {
+  const psx_f2v v = __builtin_nontemporal_load((const PSX_GLOBAL psx_f2v*)((const PSX_GLOBAL char*)base +
+                                                                            (uint64_t)(i * 8u)));
+  return make_float2(v.x, v.y);
+}
 // Streaming (non-temporal) store: the line is not kept dirty in the caches
 // for a later write-back that would compete with the next kernel's traffic.
 // Used where whole 128-byte lines are written (column tiles of 16).
