@@ -104,7 +104,9 @@ def tune(n, tsamp, d_series, accs, batches=(16, 32, 64), reps=5, limit=12, log=p
                                                            C.c_void_p, C.POINTER(C.c_int), C.c_char_p,
                                                            C.c_size_t])})
     args = W.make_search_args(n, tsamp)
-    accs = np.ascontiguousarray(accs, np.float32)
+    # every batch full, as in a search of many DM trials (short acceleration
+    # lists of short series would otherwise time part-empty batches)
+    accs = np.ascontiguousarray(np.resize(np.asarray(accs, np.float32), max(batches)))
     best = None
     for dec in [None] + candidates(n, limit):
         for b in batches:
@@ -113,11 +115,11 @@ def tune(n, tsamp, d_series, accs, batches=(16, 32, 64), reps=5, limit=12, log=p
             ns = C.c_int(8)
             desc = C.create_string_buffer(128)
             rc = P.lib().psx_batch_profile(C.byref(args), C.byref(opts), d_series, n, accs.ctypes.data,
-                                           min(b, accs.size), reps, ms.ctypes.data, C.byref(ns), desc, 128)
+                                           b, reps, ms.ctypes.data, C.byref(ns), desc, 128)
             if rc:
                 log("skip", dec, b, P.lib().psx_last_error_string().decode())
                 continue
-            per_trial = ms[:ns.value].sum() / min(b, accs.size)
+            per_trial = ms[:ns.value].sum() / b
             log("n=%d batch=%d plan=%s: %.4f ms/trial %s" % (n, b, desc.value.decode(), per_trial,
                                                            np.round(ms[:ns.value], 4).tolist()))
             if best is None or per_trial < best[0]:
