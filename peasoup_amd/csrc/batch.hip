@@ -214,11 +214,36 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   // rows (ra-1 below, ra+64 above) are computed twice.
   float2 xlo_prev = make_float2(0.f, 0.f), xhi_prev = make_float2(0.f, 0.f);
   float2 xlo_first = xlo_prev, xhi_last = xhi_prev;
+  // With the high root factor fixed, every row's loads (both bins and the
+  // low root factor) are issued before any is used: 16 rows of loads in
+  // flight per wave (127 VGPRs, the occupancy the LDS tiles allow anyway)
+  // instead of the few the compiler keeps when loads and arithmetic
+  // interleave.  The arithmetic is sweep_row's.
+  float2 zl[kWaveRows], zh[kWaveRows], wl[kWaveRows];
+  if constexpr (HI_FIXED) {
+#pragma unroll
+    for (int i = 0; i < kWaveRows; ++i) {
+      const uint32_t q = min(r0 + (uint32_t)i, half_rows);
+      const uint32_t k = q + R * e;
+      zl[i] = ldg2(y, q * L + e);
+      zh[i] = ldg2(y, (R - q) * L + emir);
+      wl[i] = ldg2(rt.lo, (2 * k > M ? M - k : k) & rt.mask);
+    }
+  }
 #pragma unroll
   for (int i = 0; i < kWaveRows; ++i) {
     const uint32_t qq = r0 + (uint32_t)i;
     float2 xlo, xhi;
-    sweep_row(qq, xlo, xhi);
+    if constexpr (HI_FIXED) {
+      const uint32_t k = min(qq, half_rows) + R * e;
+      const bool upper = 2 * k > M;
+      const float2 w = cmul(hi_e, wl[i]);
+      const float2 wm = make_float2(-w.x, w.y);
+      xlo = untangle_one(zl[i], zh[i], upper ? wm : w);
+      xhi = untangle_one(zh[i], zl[i], upper ? w : wm);
+    } else {
+      sweep_row(qq, xlo, xhi);
+    }
     if (i == 0) {
       xlo_first = xlo;
       edge_hi[wv][lane] = xhi;
