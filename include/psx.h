@@ -266,6 +266,7 @@ typedef struct psx_batch_opts {
   int32_t spectrum_layout; /* PSX_SPECTRUM_*: how the last FFT pass hands over the spectra */
   int32_t keep_engine;     /* keep the engine for the next call with the same arguments and
                               options (psx_batch_release frees it) */
+  int32_t harmonic_pass;   /* PSX_HARMONIC_*: form of the harmonic-sum pass */
 } psx_batch_opts;
 
 /* Spectrum hand-over of the batched path.  AUTO: a two-pass plan whose last
@@ -274,6 +275,16 @@ typedef struct psx_batch_opts {
  * order; ROWS: the transposed layout, an error when the plan has none.  All
  * three give identical spectra for the same FFT decomposition. */
 enum { PSX_SPECTRUM_AUTO = 0, PSX_SPECTRUM_NATURAL = 1, PSX_SPECTRUM_ROWS = 2 };
+
+/* Harmonic-sum pass of the batched path (harmonic_sum_kernel + PeakFinder,
+ * src/kernels.cu:34-100,391-423).  BOUND: a pooled half-precision copy of
+ * each spectrum (every entry rounded up from the larger of two bins) is
+ * summed in the reference's addend order; since every addend and every
+ * rounding step only grows, those sums bound the exact fold sums from
+ * above, and only bins whose bound crosses the threshold are summed exactly.
+ * EXACT: every addend of every bin gathered from the spectrum.  Both give
+ * identical candidates; BOUND moves about a third of the memory traffic. */
+enum { PSX_HARMONIC_BOUND = 0, PSX_HARMONIC_EXACT = 1 };
 
 /* Same contract as psx_worker_search, computed by the batched path: all DM
  * prologues, then every (DM, acceleration) trial through fused batches

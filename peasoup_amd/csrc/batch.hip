@@ -3,14 +3,22 @@
 // One batch = up to B acceleration trials of the same FFT length.  Per batch:
 //   k_fft_pass (resample fused into the first pass) ... FFT passes -> Z
 //   k_spec_post   untangle + interbin + normalise          Z -> P
-//   k_harm_peaks  2..2^F harmonic sums + threshold +
-//                 wave-ballot compaction                   P -> candidates
+//   k_pool_peaks  fundamental threshold + bound spectrum   P -> Q, candidates
+//   k_harm_bound  2..2^F harmonic sums over Q (an upper bound of every
+//                 fold sum); exact sums of P where a bound
+//                 crosses the threshold                    Q, P -> candidates
+// (k_harm_peaks, the exact sums of every bin, is the alternative harmonic
+// pass: same candidates, more memory traffic.)
 // The per-trial arithmetic is the plain path's, operation for operation
 // (untangled_bin, interbin, (v-mean)/std, harmonic_sums), so every value the
 // batched path thresholds equals the plain path's bit for bit; only the
 // intermediate HBM round trips (X, the interbinned spectrum before
 // normalisation, the F fold spectra) disappear.
 #include "batch.hpp"
+
+#include <cmath>
+#include <cstring>
+
 #include "common.hpp"
 #include "kernels_dev.hpp"
 
@@ -418,6 +426,290 @@ __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, 
   }
 }
 
+
+// ------------------------------------------------------------------------
+// Bound-filtered harmonic pass.
+//
+// The gather form above reads each spectrum once per fold span (8.5 spectra
+// for four folds) and is bound by the L2-miss path: every XCD's 4 MB L2
+// holds a fraction of one 16 MB spectrum, and the miss path moves about
+// 7.5 TB/s whether the MALL or HBM serves it.  Candidates are rare, so this
+// form first proves most bins empty from a smaller array:
+//   Q[u] = max(P[2u], P[2u+1]) rounded up to half precision,
+// one 16-bit entry per two bins (an eighth of the spectrum's bytes).  Every
+// addend P[x] of a fold sum is at most Q[x/2], and float addition,
+// fold_scale and the comparison are monotone (round to nearest never
+// reverses an order), so the running sum of the Q entries taken in the
+// reference's addend order is at least the exact running sum at every fold:
+// where the bound of fold k stays at or below the threshold, so does fold
+// k's exact value.  Only the bins whose bound crosses (about 1e-6 of
+// noise bins, and every real detection) are summed exactly from P with
+// harmonic_sums32, thresholded and appended, so the candidates are the
+// gather form's bit for bit.
+//
+// fold_scale<K>(v) > thr is tested as v > T[K], T[K] the largest float
+// whose fold_scale is at most thr (found on the host by bisection over the
+// float order; fold_scale is monotone), which keeps double arithmetic out
+// of the loop.
+//
+// A wave covers kBoundWaveBins consecutive bins, lane l holding bins
+// base + l + 64d (d < 8), as in the gather form.  Harmonic j/2^k of those
+// 512 bins takes its addends from at most 512j/2^k + 1 consecutive source
+// bins, one run of 256j/2^k + 1 entries of Q; the wave copies every run (and
+// the fundamental's) into its own LDS area with global_load_lds_dwordx4
+// (one instruction per run, 16 bytes per lane, no registers), waits once,
+// and then reads each addend from LDS at a constant offset per d.  The
+// wave's LDS area is private: no barriers.  Measured at 2^23 with 64 trials
+// per launch: 0.36 ms against 0.80 ms for the gather form (plus the pooling
+// pass), the same candidates.
+
+// Harmonic h (0-based, in the reference's addend order, fold 2 taking 3/4
+// before 1/4) is j/2^k with k = hK(h), j = hJ(h); fold k ends at h = 2^k - 2.
+__host__ __device__ constexpr int hK(int h) { return h < 1 ? 1 : h < 3 ? 2 : h < 7 ? 3 : h < 15 ? 4 : 5; }
+__host__ __device__ constexpr int hJ(int h) {
+  return h == 0 ? 1 : h == 1 ? 3 : h == 2 ? 1 : h < 7 ? 2 * (h - 3) + 1 : h < 15 ? 2 * (h - 7) + 1 : 2 * (h - 15) + 1;
+}
+constexpr int kBoundChunk = 8;  // entries per 16-byte chunk
+constexpr int kBoundD = kBoundWaveBins / 64;
+// chunks of harmonic h's run: its sources span at most j*W/2^k bins, W/2^k/2
+// entries, plus one entry and the chunk alignment
+__host__ __device__ constexpr int run_chunks(int h) {
+  return ((hJ(h) * (int)kBoundWaveBins >> hK(h)) / (int)kBoundPool) / kBoundChunk + 2;
+}
+// LDS offset (entries) of run h; the fundamental's run (W/2 entries + the
+// alignment) comes first, at 0
+__host__ __device__ constexpr int run_offset(int h) {
+  int r = ((int)kBoundWaveBins / (int)kBoundPool / kBoundChunk + 1) * kBoundChunk;
+  for (int g = 0; g < h; ++g) r += run_chunks(g) * kBoundChunk;
+  return r;
+}
+static_assert(run_chunks(30) <= 64 && (int)kBoundWaveBins / (int)kBoundPool / kBoundChunk + 1 <= 64,
+              "one 64-lane instruction per run");
+
+// Rounds x up to the next half-precision value (an upper bound of x).
+__device__ inline _Float16 half_up(float x) {
+  _Float16 h = (_Float16)x;
+  if ((float)h < x) {
+    uint16_t b = __builtin_bit_cast(uint16_t, h);
+    b = (b == 0x8000u) ? (uint16_t)0x0001u : (b & 0x8000u) ? (uint16_t)(b - 1u) : (uint16_t)(b + 1u);
+    h = __builtin_bit_cast(_Float16, b);
+  }
+  return h;
+}
+
+typedef float psx_f4v __attribute__((ext_vector_type(4)));
+
+// The bound spectrum Q of every trial row, and the fundamental's threshold
+// test (device_find_peaks over PeakFinder's nh = 0 window,
+// src/kernels.cu:391-423, peakfinder.hpp:78-83) on the same read of P.
+// Each lane takes four bins (one 16-byte load) and writes two entries.
+__global__ void __launch_bounds__(kB) k_pool_peaks(const float* __restrict__ P, int64_t pstride, uint32_t nb,
+                                                   Windows w, _Float16* __restrict__ Q, int64_t qstride,
+                                                   const TrialDesc* __restrict__ descs, Cand16* __restrict__ out,
+                                                   uint32_t cap, uint32_t* __restrict__ count) {
+  const uint32_t tr = blockIdx.y;
+  const float* p = P + (int64_t)tr * pstride;
+  _Float16* q = Q + (int64_t)tr * qstride;
+  const unsigned lane = threadIdx.x & 63u;
+  const uint32_t start = (uint32_t)max(w.start[0], 0), end = (uint32_t)max(w.end[0], 0);
+  const uint32_t ngroups = (nb + 3) / 4;
+  const uint32_t stride = gridDim.x * kB;
+  // whole-wave trip count, so the ballots below see every lane
+  for (uint32_t g0 = blockIdx.x * kB + (threadIdx.x & ~63u); g0 < ngroups; g0 += stride) {
+    const uint32_t g = g0 + lane;
+    const uint32_t k0 = 4 * g;
+    float x[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (g < ngroups) {
+      // k0 + 3 < pstride: rows are padded to a multiple of 64 bins
+      const psx_f4v v = __builtin_nontemporal_load((const PSX_GLOBAL psx_f4v*)(p + k0));
+      x[0] = v.x;
+      x[1] = k0 + 1 < nb ? v.y : -INFINITY;
+      x[2] = k0 + 2 < nb ? v.z : -INFINITY;
+      x[3] = k0 + 3 < nb ? v.w : -INFINITY;
+      const _Float16 q0 = half_up(fmaxf(x[0], x[1]));
+      const _Float16 q1 = half_up(fmaxf(x[2], x[3]));
+      const uint32_t pair = (uint32_t)__builtin_bit_cast(uint16_t, q0) | ((uint32_t)__builtin_bit_cast(uint16_t, q1) << 16);
+      *(PSX_GLOBAL uint32_t*)(q + 2 * g) = pair;
+    }
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) any |= x[j] > w.thr;
+    if (__ballot(any) == 0) continue;
+    const uint32_t slot = descs[tr].slot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t k = k0 + (uint32_t)j;
+      const bool hit = x[j] > w.thr && k >= start && k < end;
+      const uint64_t mask = __ballot(hit);
+      if (mask == 0) continue;
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(count, (uint32_t)__popcll(mask));
+      b = __shfl(b, 0);
+      if (hit) {
+        const uint32_t pos = b + psx::mbcnt64(mask);
+        if (pos < cap) out[pos] = Cand16{slot, 0u, (int32_t)k, x[j]};
+      }
+    }
+  }
+}
+
+struct FoldThr {
+  float t[6];  // t[k]: fold k crosses iff its running sum > t[k]
+};
+
+// Copies one run of Q (entries s0..s1, rounded out to whole chunks) into
+// the wave's LDS at dst; returns the first entry copied.
+__device__ inline uint32_t stage_run(const PSX_GLOBAL _Float16* q, uint32_t s0, uint32_t s1, _Float16* dst,
+                                     unsigned lane) {
+  const uint32_t a0 = s0 & ~(uint32_t)(kBoundChunk - 1);
+  const uint32_t n = (s1 - a0) / kBoundChunk + 1;
+  if (lane < n)
+    __builtin_amdgcn_global_load_lds((const PSX_GLOBAL void*)(q + a0 + kBoundChunk * lane),
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+  return a0;
+}
+
+template <int NH, int H>
+struct BoundRuns {
+  __device__ static void stage(const PSX_GLOBAL _Float16* q, uint32_t base, _Float16* my, uint32_t (&a0)[NH + 1],
+                               unsigned lane) {
+    if constexpr (H < NH) {
+      constexpr int K = hK(H), J = hJ(H);
+      constexpr uint32_t c = 1u << (K - 1);
+      a0[H] = stage_run(q, ((J * base + c) >> K) / kBoundPool,
+                        ((J * (base + kBoundWaveBins - 1) + c) >> K) / kBoundPool, my + run_offset(H), lane);
+      BoundRuns<NH, H + 1>::stage(q, base, my, a0, lane);
+    }
+  }
+  // adds harmonic H's bound to every running sum; at a fold's last addend
+  // marks the bins whose bound crosses that fold's threshold
+  __device__ static void add(const _Float16* my, const uint32_t (&a0)[NH + 1], uint32_t i0, const FoldThr& ft,
+                             float (&val)[kBoundD], bool (&fail)[kBoundD]) {
+    if constexpr (H < NH) {
+      constexpr int K = hK(H), J = hJ(H);
+      constexpr uint32_t c = 1u << (K - 1);
+      constexpr uint32_t step = (64u * J) >> K;  // source step between a lane's bins
+      static_assert(step % kBoundPool == 0, "constant LDS offsets per d");
+      const _Float16* b = my + run_offset(H) + (((J * i0 + c) >> K) / kBoundPool - a0[H]);
+#pragma unroll
+      for (int d = 0; d < kBoundD; ++d) val[d] += (float)b[d * (step / kBoundPool)];
+      if constexpr (((H + 2) & (H + 1)) == 0) {  // H = 2^K - 2: fold K complete
+#pragma unroll
+        for (int d = 0; d < kBoundD; ++d) fail[d] |= val[d] > ft.t[K];
+      }
+      BoundRuns<NH, H + 1>::add(my, a0, i0, ft, val, fail);
+    }
+  }
+};
+
+template <int NF>
+__global__ void __launch_bounds__(kB) k_harm_bound(const float* __restrict__ P, int64_t pstride,
+                                                   const _Float16* __restrict__ Q, int64_t qstride, Windows w,
+                                                   FoldThr ft, const TrialDesc* __restrict__ descs,
+                                                   uint32_t xcd_order, Cand16* __restrict__ out, uint32_t cap,
+                                                   uint32_t* __restrict__ count) {
+  constexpr int NH = (1 << NF) - 1;
+  constexpr int WE = run_offset(NH);  // LDS entries per wave
+  __shared__ _Float16 lds[WE * (kB / 64)];
+  uint32_t tr = blockIdx.y, bx = blockIdx.x;
+  if (xcd_order) {
+    // workgroups reach the 8 XCDs round robin: hardware block b runs on XCD
+    // b % 8, so trial t is given to XCD t % 8 whole (its bound spectrum
+    // stays in one L2); the launcher asks for this only when 8 divides the
+    // trial count
+    const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t k = b >> 3;
+    tr = (b & 7u) + 8u * (k / gridDim.x);
+    bx = k % gridDim.x;
+  }
+  const float* p = P + (int64_t)tr * pstride;
+  const PSX_GLOBAL _Float16* qg = (const PSX_GLOBAL _Float16*)(Q + (int64_t)tr * qstride);
+  const unsigned lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  _Float16* my = lds + wave * WE;
+  const uint32_t hi = (uint32_t)w.hi;
+  const uint32_t wstep = gridDim.x * (kB / 64) * kBoundWaveBins;
+  for (uint32_t base = (uint32_t)w.lo + (bx * (kB / 64) + wave) * kBoundWaveBins; base < hi; base += wstep) {
+    const uint32_t i0 = base + lane;
+    // the previous tile's LDS reads are complete before the area is refilled
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    uint32_t a0[NH + 1];
+    a0[NH] = stage_run(qg, base / kBoundPool, (base + kBoundWaveBins - 1) / kBoundPool, my, lane);
+    BoundRuns<NH, 0>::stage(qg, base, my, a0, lane);
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    float val[kBoundD];
+    bool fail[kBoundD];
+    {
+      const _Float16* b = my + (i0 / kBoundPool - a0[NH]);
+#pragma unroll
+      for (int d = 0; d < kBoundD; ++d) {
+        val[d] = (float)b[d * (64 / kBoundPool)];
+        fail[d] = false;
+      }
+    }
+    BoundRuns<NH, 0>::add(my, a0, i0, ft, val, fail);
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < kBoundD; ++d) any |= fail[d];
+    if (__ballot(any) == 0) continue;
+    // exact folds of the bins whose bound crossed (the fundamental is the
+    // pooling pass's)
+    const uint32_t slot = descs[tr].slot;
+#pragma unroll
+    for (int d = 0; d < kBoundD; ++d) {
+      if (__ballot(fail[d]) == 0) continue;
+      const uint32_t i = i0 + 64u * (uint32_t)d;
+      const bool valid = fail[d] && i < hi;
+      float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      if (valid) psx::harmonic_sums32<NF>(p, i, s);
+      bool hit[NF];
+      uint64_t mask[NF];
+      uint32_t total = 0;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        hit[k] = valid && (int32_t)i >= w.start[k + 1] && (int32_t)i < w.end[k + 1] && s[k] > w.thr;
+        mask[k] = __ballot(hit[k]);
+        total += (uint32_t)__popcll(mask[k]);
+      }
+      if (total == 0) continue;
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(count, total);
+      b = __shfl(b, 0);
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        if (hit[k]) {
+          const uint32_t pos = b + psx::mbcnt64(mask[k]);
+          if (pos < cap) out[pos] = Cand16{slot, (uint32_t)(k + 1), (int32_t)i, s[k]};
+        }
+        b += (uint32_t)__popcll(mask[k]);
+      }
+    }
+  }
+}
+
+// Largest float v with fold_scale<K>(v) <= thr, by bisection over the float
+// order (fold_scale is monotone non-decreasing).
+template <int K>
+float fold_threshold(float thr) {
+  auto from_order = [](int64_t o) {
+    const uint32_t u = o >= 0 ? (uint32_t)o : ((uint32_t)(-o) | 0x80000000u);
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  int64_t lo = -(int64_t)0x7F7FFFFF, hi = 0x7F7FFFFF;  // -FLT_MAX .. FLT_MAX
+  if (!(psx::fold_scale<K>(from_order(lo)) <= thr)) return -INFINITY;
+  if (psx::fold_scale<K>(from_order(hi)) <= thr) return INFINITY;
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (psx::fold_scale<K>(from_order(mid)) <= thr)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return from_order(lo);
+}
+
 __global__ void k_zero_u32(uint32_t* p) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *p = 0u;
 }
@@ -470,6 +762,54 @@ int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_
     default: return psx::fail(PSX_ERR_ARG, "nfolds must be 1..5");
   }
   PSX_CHECK_LAUNCH("k_harm_peaks");
+  return PSX_OK;
+}
+
+
+int64_t bound_stride(int64_t pstride) {
+  // a wave's staging reads up to kBoundWaveBins - 1 bins past the window
+  // end, plus a chunk of alignment: per-row slack covers both
+  return (pstride + kBoundWaveBins) / kBoundPool + 64;
+}
+
+int launch_pool_peaks(const float* P, int64_t pstride, uint64_t nb, const Windows& w, uint16_t* Q, int64_t qstride,
+                      const TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count,
+                      hipStream_t s) {
+  if (nb < 1 || nb > (1ull << 27) || (int64_t)nb > pstride || pstride % 4 || qstride < bound_stride(pstride))
+    return psx::fail(PSX_ERR_ARG, "pool_peaks: unsupported row layout");
+  const unsigned gx = psx::grid_for((nb + 3) / 4, kB, 1024);
+  k_pool_peaks<<<dim3(gx, ntr), kB, 0, s>>>(P, pstride, (uint32_t)nb, w, (_Float16*)Q, qstride, d_descs, out, cap,
+                                            count);
+  PSX_CHECK_LAUNCH("k_pool_peaks");
+  return PSX_OK;
+}
+
+int launch_harm_bound(const float* P, int64_t pstride, const uint16_t* Q, int64_t qstride, const Windows& w,
+                      uint32_t nfolds, const TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap,
+                      uint32_t* count, hipStream_t s) {
+  if (w.hi <= w.lo) return PSX_OK;
+  if (w.hi > (1ll << 27)) return psx::fail(PSX_ERR_UNSUPPORTED, "harm_bound: more than 2^27 bins");
+  if (w.lo < 0 || w.hi > pstride || qstride < bound_stride(pstride))
+    return psx::fail(PSX_ERR_ARG, "harm_bound: window outside the spectrum row");
+  FoldThr ft{};
+  ft.t[1] = fold_threshold<1>(w.thr);
+  ft.t[2] = fold_threshold<2>(w.thr);
+  ft.t[3] = fold_threshold<3>(w.thr);
+  ft.t[4] = fold_threshold<4>(w.thr);
+  ft.t[5] = fold_threshold<5>(w.thr);
+  const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), (uint64_t)kBoundWaveBins * (kB / 64), 2048);
+  const uint32_t xcd = ntr % 8 == 0 ? 1u : 0u;
+  const dim3 g(gx, ntr);
+  const _Float16* q = (const _Float16*)Q;
+  switch (nfolds) {
+    case 1: k_harm_bound<1><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
+    case 2: k_harm_bound<2><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
+    case 3: k_harm_bound<3><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
+    case 4: k_harm_bound<4><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
+    case 5: k_harm_bound<5><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
+    default: return psx::fail(PSX_ERR_ARG, "nfolds must be 1..5");
+  }
+  PSX_CHECK_LAUNCH("k_harm_bound");
   return PSX_OK;
 }
 

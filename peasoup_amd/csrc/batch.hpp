@@ -41,6 +41,21 @@ constexpr uint32_t kHarmWaveBins = 512;
 int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_t nfolds,
                       const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count,
                       hipStream_t s);
+// Bound-filtered harmonic pass.  Q holds, per trial row of qstride 16-bit
+// entries, half-precision upper bounds of bin pairs (entry u >= P[2u] and
+// P[2u+1], rounded up); launch_pool_peaks writes it from P together with
+// the fundamental's above-threshold bins, launch_harm_bound appends the
+// folds' (nh >= 1) from Q, summing P exactly only where a bound crosses.
+constexpr uint32_t kBoundPool = 2;         // bins per entry of Q
+constexpr uint32_t kBoundWaveBins = 512;   // bins per wave of k_harm_bound
+// Row stride (entries) of Q for spectrum rows of pstride floats.
+int64_t bound_stride(int64_t pstride);
+int launch_pool_peaks(const float* P, int64_t pstride, uint64_t nb, const Windows& w, uint16_t* Q, int64_t qstride,
+                      const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count,
+                      hipStream_t s);
+int launch_harm_bound(const float* P, int64_t pstride, const uint16_t* Q, int64_t qstride, const Windows& w,
+                      uint32_t nfolds, const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap,
+                      uint32_t* count, hipStream_t s);
 int launch_zero(uint32_t* p, hipStream_t s);
 
 }  // namespace batch

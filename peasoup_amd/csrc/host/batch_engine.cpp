@@ -41,6 +41,7 @@ uint64_t Engine::device_bytes(uint64_t N, uint32_t B, uint32_t cap) {
   uint64_t b = 0;
   b += 2 * M * B * sizeof(float2);      // FFT work + Z
   b += (pst * B + kHarmWaveBins) * sizeof(float);  // P
+  b += (uint64_t)bound_stride((int64_t)pst) * B * sizeof(uint16_t);  // Q
   b += 2 * (uint64_t)cap * sizeof(Cand16);
   b += nb * (sizeof(float2) + 2 * sizeof(float)) + nb / 4 * sizeof(float);  // prologue X, A, median
   b += 2 * M * sizeof(float2) + 16 * 1024 * 1024;                          // prologue plans + tables
@@ -62,8 +63,18 @@ Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t s
   if (opt.spectrum_layout == PSX_SPECTRUM_ROWS && !rows_ok)
     throw std::runtime_error("spectrum_layout ROWS: this FFT plan has no row-layout last pass");
   rows_out = rows_ok && opt.spectrum_layout != PSX_SPECTRUM_NATURAL;
+  if (opt.harmonic_pass != PSX_HARMONIC_BOUND && opt.harmonic_pass != PSX_HARMONIC_EXACT)
+    throw std::runtime_error("harmonic_pass must be PSX_HARMONIC_BOUND or PSX_HARMONIC_EXACT");
+  bound = opt.harmonic_pass == PSX_HARMONIC_BOUND && F > 0;
   dalloc(&Z, M * B);
   dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
+  if (bound) {
+    // the pooling pass writes each row's bounds up to bin nb; the slack past
+    // them (read only for bins past the window) holds -inf
+    qstride = bound_stride(pstride);
+    dalloc(&Q, (uint64_t)qstride * B);
+    hip_ok(hipMemsetD16(Q, 0xFC00u, (size_t)qstride * B), "bound spectrum fill");
+  }
   for (Slot& sl : slots) {
     dalloc(&sl.d_desc, B);
     halloc(&sl.h_desc, B);
@@ -119,6 +130,7 @@ Engine::~Engine() {
   }
   (void)hipFree(Z);
   (void)hipFree(P);
+  (void)hipFree(Q);
   psx_fft_plan_destroy(plan);
   psx_fft_plan_destroy(p_r2c);
   psx_fft_plan_destroy(p_c2r);
@@ -162,6 +174,20 @@ void Engine::spectrum(Slot& sl, uint32_t n) {
     psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s), "spec_rows");
   else
     psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
+}
+
+// Harmonic pass: the bound-filtered form (pooling pass with the
+// fundamental's threshold, then the folds from the bounds) or the exact
+// gathers of every addend.
+void Engine::harmonics(Slot& sl, uint32_t n) {
+  if (bound) {
+    psx_ok(launch_pool_peaks(P, pstride, nb, win, Q, qstride, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
+           "pool_peaks");
+    psx_ok(launch_harm_bound(P, pstride, Q, qstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
+           "harm_bound");
+  } else {
+    psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
+  }
 }
 
 void Engine::prologue_async(const float* d_series, uint64_t n_real, float* d_tim, PrologueResult* res,
@@ -209,7 +235,7 @@ void Engine::launch(Slot& sl, uint32_t n, bool run) {
     psx_ok(launch_zero(sl.d_count, s), "zero");
     psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
     spectrum(sl, n);
-    psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
+    harmonics(sl, n);
     hip_ok(hipMemcpyAsync(sl.h_count, sl.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count copy");
     hip_ok(hipMemcpyAsync(sl.h_cands, sl.d_cands, kPrefix * sizeof(Cand16), hipMemcpyDeviceToHost, s),
            "candidate copy");
@@ -405,12 +431,19 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
   constexpr int kRun = 4;
   auto runs = [&](int k) { return k < np && psx::fft::pass_in_place(plan, k) ? 1 : kRun; };
   auto stage = [&](int k) {
-    if (k < np)
+    if (k < np) {
       psx_ok(psx::fft::exec_fused_pass(plan, k, sl.d_desc, n, Z, s, rows_out), "fused pass");
-    else if (k == np)
+    } else if (k == np) {
       spectrum(sl, n);
-    else
+      if (bound)
+        psx_ok(launch_pool_peaks(P, pstride, nb, win, Q, qstride, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
+               "pool_peaks");
+    } else if (bound) {
+      psx_ok(launch_harm_bound(P, pstride, Q, qstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
+             "harm_bound");
+    } else {
       psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
+    }
   };
   for (int r = 0; r < reps + 1; ++r) {
     psx_ok(launch_zero(sl.d_count, s), "zero");

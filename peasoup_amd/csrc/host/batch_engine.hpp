@@ -46,10 +46,14 @@ struct EngineOptions {
   bool use_graph = true;
   const psx::fft::Decomp* decomp = nullptr;  // FFT decomposition override (autotuner)
   int spectrum_layout = PSX_SPECTRUM_AUTO;
+  int harmonic_pass = PSX_HARMONIC_BOUND;
 };
 
 struct StageTimes {
-  std::vector<double> ms;  // per launch: FFT passes (first one with resampling), spec_post, harm_peaks
+  // per launch: FFT passes (first one with resampling), spectrum pass (with
+  // the bound pooling and the fundamental's threshold in the bound form),
+  // harmonic pass
+  std::vector<double> ms;
   int npass = 0;
   std::string desc;  // FFT decomposition
 };
@@ -130,6 +134,7 @@ class Engine {
   // enqueue's body; run = false only builds the slot's graph for n
   void launch(Slot& sl, uint32_t n, bool run);
   void spectrum(Slot& sl, uint32_t n);
+  void harmonics(Slot& sl, uint32_t n);
   // the fused FFT leaves the transposed row layout (rows_R rows of rows_L)
   bool rows_out = false;
   uint32_t rows_R = 0, rows_L = 0;
@@ -146,6 +151,10 @@ class Engine {
   psx::fft::RootTable rt{};
   float2* Z = nullptr;
   float* P = nullptr;
+  // bound-filtered harmonic pass: Q rows of qstride half-precision bounds
+  bool bound = true;
+  int64_t qstride = 0;
+  uint16_t* Q = nullptr;
   Windows win{};
   float factor[6];
   Slot slots[2];

@@ -104,6 +104,7 @@ bool engine_options(const psx_batch_opts* opts, EngineOptions& eo, psx::fft::Dec
     eo.cand_capacity = opts->cand_capacity ? opts->cand_capacity : eo.cand_capacity;
     eo.use_graph = opts->use_graph != 0;
     eo.spectrum_layout = opts->spectrum_layout;
+    eo.harmonic_pass = opts->harmonic_pass;
     if (opts->fft_npass > 0) {
       forced.npass = opts->fft_npass;
       for (int q = 0; q < 3; ++q) {
@@ -126,6 +127,7 @@ struct EngineKey {
   uint32_t batch = 0, cand_capacity = 0;
   bool use_graph = false, forced = false;
   int spectrum_layout = 0;
+  int harmonic_pass = 0;
   psx::fft::Decomp decomp;
 };
 EngineKey engine_key(const psx_search_args& a, const EngineOptions& eo) {
@@ -137,6 +139,7 @@ EngineKey engine_key(const psx_search_args& a, const EngineOptions& eo) {
   k.cand_capacity = eo.cand_capacity;
   k.use_graph = eo.use_graph;
   k.spectrum_layout = eo.spectrum_layout;
+  k.harmonic_pass = eo.harmonic_pass;
   k.forced = eo.decomp != nullptr;
   if (eo.decomp) k.decomp = *eo.decomp;
   return k;
@@ -144,7 +147,7 @@ EngineKey engine_key(const psx_search_args& a, const EngineOptions& eo) {
 bool same_key(const EngineKey& x, const EngineKey& y) {
   return x.device == y.device && std::memcmp(&x.args, &y.args, sizeof(x.args)) == 0 && x.batch == y.batch &&
          x.cand_capacity == y.cand_capacity && x.use_graph == y.use_graph && x.forced == y.forced &&
-         x.spectrum_layout == y.spectrum_layout &&
+         x.spectrum_layout == y.spectrum_layout && x.harmonic_pass == y.harmonic_pass &&
          (!x.forced || (x.decomp.npass == y.decomp.npass &&
                         std::memcmp(x.decomp.L, y.decomp.L, sizeof(x.decomp.L)) == 0 &&
                         std::memcmp(x.decomp.C, y.decomp.C, sizeof(x.decomp.C)) == 0));

@@ -97,9 +97,10 @@ __device__ inline float stretch_at(const float* __restrict__ in, uint64_t in_cou
 }
 
 // Scale of fold K's running sum (1-based): times 2^-K/2, computed in double
-// for odd K as the reference does (src/kernels.cu:38-98).
+// for odd K as the reference does (src/kernels.cu:38-98).  Host and device
+// round every step alike (IEEE multiply and conversion, nearest even).
 template <int K>
-__device__ inline float fold_scale(float val) {
+__host__ __device__ inline float fold_scale(float val) {
   if constexpr (K == 1) return (float)((double)val * 0.70710678118654752440);
   else if constexpr (K == 2) return val * 0.5f;
   else if constexpr (K == 3) return (float)((double)val * 0.35355339059327376220);

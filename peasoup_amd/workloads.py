@@ -170,17 +170,19 @@ def generate_host(w: Workload, trials=None):
 class BatchOpts(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("cand_capacity", C.c_uint32), ("use_graph", C.c_int32),
                 ("fft_npass", C.c_int32), ("fft_len", C.c_int32 * 3), ("fft_cols", C.c_int32 * 3),
-                ("spectrum_layout", C.c_int32), ("keep_engine", C.c_int32)]
+                ("spectrum_layout", C.c_int32), ("keep_engine", C.c_int32), ("harmonic_pass", C.c_int32)]
 
 
 SPECTRUM_AUTO, SPECTRUM_NATURAL, SPECTRUM_ROWS = 0, 1, 2
+HARMONIC_BOUND, HARMONIC_EXACT = 0, 1
 
 
 def make_batch_opts(batch=16, cand_capacity=0, use_graph=True, decomp=None, spectrum_layout=SPECTRUM_AUTO,
-                    keep_engine=False):
+                    keep_engine=False, harmonic_pass=HARMONIC_BOUND):
     o = BatchOpts(batch, cand_capacity, 1 if use_graph else 0, 0)
     o.spectrum_layout = spectrum_layout
     o.keep_engine = 1 if keep_engine else 0
+    o.harmonic_pass = harmonic_pass
     if decomp:
         o.fft_npass = len(decomp)
         for q, (L, Cc) in enumerate(decomp):
