@@ -36,15 +36,23 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n, npass):
+def stage_bytes(n, npass, bound=True):
     """Algorithmic HBM bytes per trial of each fused launch: FFT pass 0 reads
     the n-sample series through the resampler and writes n/2 complex; every
     other FFT pass reads and writes n/2 complex; the spectrum pass reads n/2
-    complex and writes n/2+1 floats; the harmonic/threshold pass reads n/2+1
-    floats (its gathers hit the caches; candidates are negligible)."""
+    complex and writes n/2+1 floats.  Bound-filtered harmonic form: the
+    pooling pass (timed with the spectrum pass) reads the n/2+1 floats again
+    and writes one 16-bit bound per two bins, and the harmonic pass reads the
+    bounds once (its run copies hit the caches; the exact sums of the rare
+    crossing bins and the candidates are negligible).  Exact-gather form: the
+    harmonic pass reads the n/2+1 floats once."""
     m, nb = n // 2, n // 2 + 1
+    qb = 2 * ((nb + 1) // 2)
     b = [4 * n + 8 * m] + [16 * m] * (npass - 1)
-    b += [8 * m + 4 * nb, 4 * nb]
+    if bound:
+        b += [8 * m + 4 * nb + 4 * nb + qb, qb]
+    else:
+        b += [8 * m + 4 * nb, 4 * nb]
     return b
 
 
@@ -271,6 +279,8 @@ def main():
     ap.add_argument("--ndm", type=int, default=64)
     ap.add_argument("--batch", type=int, default=0, help="trials per fused batch (0 = tuned/default)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--harmonic", choices=("bound", "exact"), default="bound",
+                    help="harmonic pass: bound-filtered (default) or exact gathers of every addend")
     ap.add_argument("--no-keep", action="store_true",
                     help="build and free the search engine in every step instead of keeping it between steps")
     ap.add_argument("--no-cpu", action="store_true")
@@ -306,7 +316,9 @@ def main():
     ntrials = w.ndm * accs.size
     tuned = tuning.lookup(w.size, dev["name"])
     batch = a.batch or tuned.get("batch", 16)
-    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"), keep_engine=not a.no_keep)
+    bound = a.harmonic == "bound"
+    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"), keep_engine=not a.no_keep,
+                             harmonic_pass=W.HARMONIC_BOUND if bound else W.HARMONIC_EXACT)
     log("workload %s: %d DMs x %d accels = %d trials, batch %d, decomp %s" %
         (w.name, w.ndm, accs.size, ntrials, batch, tuned.get("decomp")))
 
@@ -355,8 +367,9 @@ def main():
     stage_ms, desc = profile_stages(P, W, args_c, opts, d_series.ptr, w.size, accs[:batch], reps=10)
     nb_trials = min(batch, accs.size)
     npass = len(stage_ms) - 2
-    sbytes = [b * nb_trials for b in stage_bytes(w.size, npass)]
-    names = ["fft_pass0_resample"] + ["fft_pass%d" % q for q in range(1, npass)] + ["spec_post", "harm_peaks"]
+    sbytes = [b * nb_trials for b in stage_bytes(w.size, npass, bound)]
+    names = (["fft_pass0_resample"] + ["fft_pass%d" % q for q in range(1, npass)] +
+             (["spec_post+pool_peaks", "harm_bound"] if bound else ["spec_post", "harm_peaks"]))
     stages = {nm: {"ms": round(ms, 4), "GBps": round(bb / (ms * 1e-3) / 1e9, 1), "bytes": bb}
               for nm, ms, bb in zip(names, stage_ms, sbytes)}
     dom = int(np.argmax(stage_ms))
@@ -376,6 +389,7 @@ def main():
                                % (w.ndm, len(w.injections)),
                    "fft_size": w.size, "ndm": w.ndm, "trials_per_step": ntrials, "batch": batch,
                    "fft_plan": desc, "graphs": not a.no_graph, "engine_kept_between_steps": not a.no_keep,
+                   "harmonic_pass": a.harmonic,
                    "device": dev["name"]},
         "roofline": roofline, "stages": stages,
         "algorithmic_GBps": round(value * (12 * w.size + 68 * (w.size // 2 + 1)) / 1e9, 1),

@@ -107,6 +107,10 @@ def stage_of(name):
         return "spec_post"
     if "k_harm_peaks" in name:
         return "harm_peaks"
+    if "k_pool_peaks" in name:
+        return "pool_peaks"
+    if "k_harm_bound" in name:
+        return "harm_bound"
     return None
 
 

@@ -23,12 +23,19 @@ def test_stage_bytes_model():
     import bench
     n = 1 << 23
     m, nb = n // 2, n // 2 + 1
-    two = bench.stage_bytes(n, 2)
+    two = bench.stage_bytes(n, 2, bound=False)
     assert two == [4 * n + 8 * m, 16 * m, 8 * m + 4 * nb, 4 * nb]
-    three = bench.stage_bytes(n, 3)
+    three = bench.stage_bytes(n, 3, bound=False)
     assert three[:3] == [4 * n + 8 * m, 16 * m, 16 * m] and three[3:] == two[2:]
     # the fused trial moves 24n bytes with two FFT passes (about 28n with three)
     assert abs(sum(two) - 24 * n) <= 16
+    # bound-filtered harmonic form: the pooling pass reads the spectrum again
+    # and writes one 16-bit bound per two bins (n/2 + 2 bytes), which the
+    # harmonic pass reads once
+    qb = nb + 1
+    bound = bench.stage_bytes(n, 2)
+    assert bound[:2] == two[:2]
+    assert bound[2] == 8 * m + 8 * nb + qb and bound[3] == qb
 
 
 def test_cpu_baseline_threads_and_model():

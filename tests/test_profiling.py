@@ -22,6 +22,8 @@ def test_stage_labels():
     assert pr.stage_of("batch::k_spec_rows") == "spec_post"
     assert pr.stage_of("batch::k_spec_edge") is None
     assert pr.stage_of("batch::k_harm_peaks<4>") == "harm_peaks"
+    assert pr.stage_of("batch::k_pool_peaks") == "pool_peaks"
+    assert pr.stage_of("batch::k_harm_bound<4>") == "harm_bound"
     assert pr.stage_of("k_convert<float>") is None
 
 
