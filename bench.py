@@ -36,21 +36,22 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n, npass, bound=True):
+def stage_bytes(n, npass, bound=True, fused=True):
     """Algorithmic HBM bytes per trial of each fused launch: FFT pass 0 reads
     the n-sample series through the resampler and writes n/2 complex; every
     other FFT pass reads and writes n/2 complex; the spectrum pass reads n/2
     complex and writes n/2+1 floats.  Bound-filtered harmonic form: the
-    pooling pass (timed with the spectrum pass) reads the n/2+1 floats again
-    and writes one 16-bit bound per two bins, and the harmonic pass reads the
-    bounds once (its run copies hit the caches; the exact sums of the rare
-    crossing bins and the candidates are negligible).  Exact-gather form: the
-    harmonic pass reads the n/2+1 floats once."""
+    spectrum stage also writes one 16-bit bound per two bins (fused into the
+    row-layout spectrum pass; after the natural layout a pooling pass reads
+    the n/2+1 floats again to do it), and the harmonic pass reads the bounds
+    once (its run copies hit the caches; the exact sums of the rare crossing
+    bins and the candidates are negligible).  Exact-gather form: the harmonic
+    pass reads the n/2+1 floats once."""
     m, nb = n // 2, n // 2 + 1
     qb = 2 * ((nb + 1) // 2)
     b = [4 * n + 8 * m] + [16 * m] * (npass - 1)
     if bound:
-        b += [8 * m + 4 * nb + 4 * nb + qb, qb]
+        b += [8 * m + 4 * nb + qb + (0 if fused else 4 * nb), qb]
     else:
         b += [8 * m + 4 * nb, 4 * nb]
     return b
@@ -367,9 +368,9 @@ def main():
     stage_ms, desc = profile_stages(P, W, args_c, opts, d_series.ptr, w.size, accs[:batch], reps=10)
     nb_trials = min(batch, accs.size)
     npass = len(stage_ms) - 2
-    sbytes = [b * nb_trials for b in stage_bytes(w.size, npass, bound)]
+    sbytes = [b * nb_trials for b in stage_bytes(w.size, npass, bound, fused=desc.endswith(" rows"))]
     names = (["fft_pass0_resample"] + ["fft_pass%d" % q for q in range(1, npass)] +
-             (["spec_post+pool_peaks", "harm_bound"] if bound else ["spec_post", "harm_peaks"]))
+             (["spectrum", "harm_bound"] if bound else ["spec_post", "harm_peaks"]))
     stages = {nm: {"ms": round(ms, 4), "GBps": round(bb / (ms * 1e-3) / 1e9, 1), "bytes": bb}
               for nm, ms, bb in zip(names, stage_ms, sbytes)}
     dom = int(np.argmax(stage_ms))

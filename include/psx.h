@@ -277,7 +277,7 @@ typedef struct psx_batch_opts {
 enum { PSX_SPECTRUM_AUTO = 0, PSX_SPECTRUM_NATURAL = 1, PSX_SPECTRUM_ROWS = 2 };
 
 /* Harmonic-sum pass of the batched path (harmonic_sum_kernel + PeakFinder,
- * src/kernels.cu:34-100,391-423).  BOUND: a pooled half-precision copy of
+ * src/kernels.cu:34-100,391-423).  BOUND: a pooled bfloat16 copy of
  * each spectrum (every entry rounded up from the larger of two bins) is
  * summed in the reference's addend order; since every addend and every
  * rounding step only grows, those sums bound the exact fold sums from
@@ -312,6 +312,18 @@ int psx_batch_release(void);
 int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
                       uint64_t nsamps, const float* accs, uint32_t naccs, int reps, double* stage_ms, int* nstage,
                       char* desc, size_t cap);
+
+/* Self-check of the bound-filtered harmonic pass's input (no counterpart
+ * in the reference): runs the prologue of d_series and the FFT and spectrum
+ * stages of one batch (min(naccs, batch) trials), then recomputes on the host,
+ * from the spectra P, every bound entry (the bfloat16 value rounded up
+ * from max(P[2u], P[2u+1])) and the fundamental's above-threshold bins.
+ * result[0] = entries checked, [1] = entries that differ (0 when correct),
+ * [2] = fundamental bins above threshold on the host, [3] = candidates the
+ * device appended (equal to [2] when correct).  For tests and for validating
+ * a new FFT decomposition. */
+int psx_batch_bound_audit(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
+                          uint64_t nsamps, const float* accs, uint32_t naccs, uint64_t* result);
 
 /* ------------------------------------------------------------------ */
 /* Serving: continuous batching of segment-search requests             */

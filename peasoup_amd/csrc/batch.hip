@@ -52,6 +52,47 @@ __device__ inline float2 shfl_f2(float2 v, int src) {
 
 constexpr uint32_t kStep = 63;  // pairs written per wave
 
+// bfloat16 bits (the top half of the float) of the smallest bfloat16 >= x:
+// a positive float with any of its low 16 bits set rounds its top half up
+// (a carry into the exponent is the next binade, and past the largest
+// finite value +inf); a negative one truncates toward zero, which is up.
+__device__ inline uint16_t bf16_up(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (uint16_t)(((int32_t)u < 0 ? u : u + 0xFFFFu) >> 16);
+}
+__device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Outputs of the spectrum pass for the bound-filtered harmonic pass: the
+// bound spectrum (one bfloat16 upper bound per bin pair 2u, 2u+1) and
+// the fundamental's above-threshold bins (device_find_peaks over
+// PeakFinder's nh = 0 window, src/kernels.cu:391-423, peakfinder.hpp:78-83).
+struct BoundOut {
+  uint16_t* Q;
+  int64_t qstride;
+  int32_t start0, end0;
+  float thr;
+  Cand16* out;
+  uint32_t cap;
+  uint32_t* count;
+};
+
+// Appends the calling lanes' hits (bin k, value v) as fundamental
+// candidates of slot; every lane of the wave calls it, lane 0 included.
+__device__ inline void append_fundamental(bool hit, uint32_t slot, uint32_t k, float v, const BoundOut& bo) {
+  const uint64_t mask = __ballot(hit);
+  if (mask == 0) return;
+  uint32_t b = 0;
+  if ((threadIdx.x & 63u) == 0) b = atomicAdd(bo.count, (uint32_t)__popcll(mask));
+  b = __shfl(b, 0);
+  if (hit) {
+    const uint32_t pos = b + psx::mbcnt64(mask);
+    if (pos < bo.cap) bo.out[pos] = Cand16{slot, 0u, (int32_t)k, v};
+  }
+}
+__device__ inline bool fundamental_hit(float v, uint32_t k, const BoundOut& bo) {
+  return v > bo.thr && (int32_t)k >= bo.start0 && (int32_t)k < bo.end0;
+}
+
 template <int Q>
 __global__ void __launch_bounds__(kB) k_spec_post(const float2* __restrict__ Z, float* __restrict__ P, uint64_t M,
                                                   int64_t pstride, RootTable rt, const TrialDesc* __restrict__ descs) {
@@ -151,11 +192,15 @@ __device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_
   untangle_pair(z_rows(y, R, L, k), z_rows(y, R, L, M - k), k, M, rt, xk, xm);
   return xk;
 }
-// t < L: row 0; t < 2L: row R/2; t == 2L: bin M
-__device__ inline void spec_edge_bin(const float2* y, float* p, uint32_t R, uint32_t L, uint32_t t,
-                                     const RootTable& rt, float mean_n, float std_n) {
+// t < L: row 0; t < 2L: row R/2; t == 2L: bin M.  Returns the bin and its
+// value; with bo, also the bound of the bin pair (k, k+1), whose second bin
+// (row 1 or R/2 + 1, a tile's) is recomputed here through the same untangle
+// and interbin arithmetic, so it equals the tile's value bit for bit.
+template <bool BOUND>
+__device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uint32_t L, uint32_t t,
+                                      const RootTable& rt, float mean_n, float std_n, uint16_t* q, uint32_t& k) {
   const uint32_t M = R * L;
-  const uint32_t k = t < L ? R * t : (t < 2 * L ? R / 2 + R * (t - L) : M);
+  k = t < L ? R * t : (t < 2 * L ? R / 2 + R * (t - L) : M);
   float2 x, prev = make_float2(0.f, 0.f);
   if (k == M) {
     // X[M] is the upper bin of pair 0
@@ -165,13 +210,20 @@ __device__ inline void spec_edge_bin(const float2* y, float* p, uint32_t R, uint
     x = x_rows(y, R, L, k, rt);
   }
   if (k > 0) prev = x_rows(y, R, L, k - 1, rt);
-  p[k] = normalised(x, prev, mean_n, std_n);
+  const float v = normalised(x, prev, mean_n, std_n);
+  p[k] = v;
+  if constexpr (BOUND) {
+    float m = v;  // bin M has no partner (n/2 + 1 bins)
+    if (k < M) m = fmaxf(v, normalised(x_rows(y, R, L, k + 1, rt), x, mean_n, std_n));
+    q[k / 2] = bf16_up(m);
+  }
+  return v;
 }
 
-template <bool HI_FIXED>
+template <bool HI_FIXED, bool BOUND>
 __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
                                                   uint32_t L, int64_t pstride, RootTable rt,
-                                                  const TrialDesc* __restrict__ descs) {
+                                                  const TrialDesc* __restrict__ descs, BoundOut bo) {
   __shared__ float lo_t[kColTile][kRowTile + 1];
   __shared__ float up_t[kColTile][kRowTile + 1];
   __shared__ float2 edge_lo[kB / 64][kColTile];  // X of each wave's last lower row
@@ -181,10 +233,15 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   const float2* y = Y + (uint64_t)tr * M;
   float* p = P + (int64_t)tr * pstride;
   const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
+  uint16_t* q = BOUND ? bo.Q + (int64_t)tr * bo.qstride : nullptr;
   if (blockIdx.y == gridDim.y - 1) {
     // the edge bins: L/64 workgroups of kB threads cover the 2L+1 of them
     const uint32_t t = blockIdx.x * kB + threadIdx.x;
-    if (t <= 2 * L) spec_edge_bin(y, p, R, L, t, rt, mean_n, std_n);
+    const bool valid = t <= 2 * L;
+    uint32_t k = 0;
+    float v = 0.f;
+    if (valid) v = spec_edge_bin<BOUND>(y, p, R, L, t, rt, mean_n, std_n, q, k);
+    if constexpr (BOUND) append_fundamental(valid && fundamental_hit(v, k, bo), descs[tr].slot, k, v, bo);
     return;
   }
   const uint32_t ra = 1 + blockIdx.y * kRowTile, ea = blockIdx.x * kColTile;
@@ -264,22 +321,76 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   }
   edge_lo[wv][lane] = xlo_prev;
   xhi_last = xhi_prev;
-  float2 below = make_float2(0.f, 0.f), above = make_float2(0.f, 0.f), unused;
-  if (wv == 0) pair_row(ra - 1, below, unused);
-  if (wv == kB / 64 - 1) pair_row(ra + kRowTile, unused, above);
+  float2 below = make_float2(0.f, 0.f), above = make_float2(0.f, 0.f), next_lo;
+  if (wv == 0) pair_row(ra - 1, below, next_lo);
+  if (wv == kB / 64 - 1) pair_row(ra + kRowTile, next_lo, above);
   __syncthreads();
   if (wv > 0) below = edge_lo[wv - 1][lane];
   if (wv < kB / 64 - 1) above = edge_hi[wv + 1][lane];
   lo_t[lane][slot(r0)] = normalised(xlo_first, below, mean_n, std_n);
   up_t[lane][slot(r0 + kWaveRows - 1)] = normalised(xhi_last, above, mean_n, std_n);
+  if constexpr (BOUND) {
+    // P of the next tile's first row (ra + 64), the partner of this tile's
+    // last row in a bound pair: its X was loaded for the upper halo, and the
+    // interbin neighbour is this tile's last row.  Slot kRowTile is free in
+    // every tile but the last, which has no such pair.
+    if (wv == kB / 64 - 1 && ra + kRowTile < half_rows)
+      lo_t[lane][kRowTile] = normalised(next_lo, xlo_prev, mean_n, std_n);
+  }
   __syncthreads();
   const uint32_t rr = threadIdx.x & 63u;
   const uint32_t r = ra + rr;
-  if (r < half_rows) {
+  if constexpr (!BOUND) {
+    if (r < half_rows) {
+#pragma unroll 4
+      for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
+        stg1(p, r + R * (ea + j), lo_t[j][rr]);
+        stg1(p, (R - r) + R * (L - 1 - (ea + j)), up_t[j][rr]);
+      }
+    }
+  } else {
+    // Bound pairs (2u, 2u+1): rows 0/1 and R/2 / R/2+1 of every column are the
+    // edge workgroups'; a tile's column holds 32 lower pairs, rows
+    // (ra+2i+1, ra+2i+2), i < 32 (slot kRowTile for the last one's second
+    // row), and 32 upper pairs, rows R-r for r = ra+63-2i and ra+62-2i.
+    // Lane i < 32 bounds lower pair i, lane 32+i upper pair i: one 16-bit
+    // store covers both runs of 32 entries.  Every value written is also
+    // tested as a fundamental.
+    const bool rv = r < half_rows;
+    const bool lower = rr < 32;
+    const uint32_t i = rr & 31u;
+    const uint32_t s0 = lower ? 2 * i + 1 : 63 - 2 * i;  // slot of the pair's even row
+    const uint32_t s1 = lower ? s0 + 1 : s0 - 1;         // and of its odd row
+    const uint32_t rpair = ra + s0;                      // lower-half row of the even-row slot
+    const bool pv = rpair < half_rows;
+    bool any = false;
 #pragma unroll 4
     for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
-      stg1(p, r + R * (ea + j), lo_t[j][rr]);
-      stg1(p, (R - r) + R * (L - 1 - (ea + j)), up_t[j][rr]);
+      const uint32_t kl = r + R * (ea + j);
+      const uint32_t ku = (R - r) + R * (L - 1 - (ea + j));
+      const float vl = lo_t[j][rr], vu = up_t[j][rr];
+      if (rv) {
+        stg1(p, kl, vl);
+        stg1(p, ku, vu);
+        any |= fmaxf(vl, vu) > bo.thr;
+      }
+      if (pv) {
+        const float m = lower ? fmaxf(lo_t[j][s0], lo_t[j][s1]) : fmaxf(up_t[j][s0], up_t[j][s1]);
+        const uint32_t k2 = lower ? rpair + R * (ea + j) : (R - rpair) + R * (L - 1 - (ea + j));
+        q[k2 / 2] = bf16_up(m);
+      }
+    }
+    // above-threshold values are rare: a second sweep over the tile appends
+    // them, in the waves that have any
+    if (__ballot(any) != 0) {
+      const uint32_t slot_id = descs[tr].slot;
+      for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
+        const uint32_t kl = r + R * (ea + j);
+        const uint32_t ku = (R - r) + R * (L - 1 - (ea + j));
+        const float vl = lo_t[j][rr], vu = up_t[j][rr];
+        append_fundamental(rv && fundamental_hit(vl, kl, bo), slot_id, kl, vl, bo);
+        append_fundamental(rv && fundamental_hit(vu, ku, bo), slot_id, ku, vu, bo);
+      }
     }
   }
 }
@@ -435,8 +546,8 @@ __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, 
 // holds a fraction of one 16 MB spectrum, and the miss path moves about
 // 7.5 TB/s whether the MALL or HBM serves it.  Candidates are rare, so this
 // form first proves most bins empty from a smaller array:
-//   Q[u] = max(P[2u], P[2u+1]) rounded up to half precision,
-// one 16-bit entry per two bins (an eighth of the spectrum's bytes).  Every
+//   Q[u] = max(P[2u], P[2u+1]) rounded up to bfloat16,
+// one 16-bit entry per two bins (a quarter of the spectrum's bytes).  Every
 // addend P[x] of a fold sum is at most Q[x/2], and float addition,
 // fold_scale and the comparison are monotone (round to nearest never
 // reverses an order), so the running sum of the Q entries taken in the
@@ -445,7 +556,11 @@ __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, 
 // k's exact value.  Only the bins whose bound crosses (about 1e-6 of
 // noise bins, and every real detection) are summed exactly from P with
 // harmonic_sums32, thresholded and appended, so the candidates are the
-// gather form's bit for bit.
+// gather form's bit for bit.  bfloat16 (8 significant bits) loosens the
+// bound no more than half precision does at these values (the same rate of
+// crossing noise bins in a float64 simulation), and rounds up with two
+// integer operations where half precision needs a conversion, a check and
+// a step.
 //
 // fold_scale<K>(v) > thr is tested as v > T[K], T[K] the largest float
 // whose fold_scale is at most thr (found on the host by bisection over the
@@ -460,8 +575,8 @@ __global__ void __launch_bounds__(kB) k_harm_peaks(const float* __restrict__ P, 
 // (one instruction per run, 16 bytes per lane, no registers), waits once,
 // and then reads each addend from LDS at a constant offset per d.  The
 // wave's LDS area is private: no barriers.  Measured at 2^23 with 64 trials
-// per launch: 0.36 ms against 0.80 ms for the gather form (plus the pooling
-// pass), the same candidates.
+// per launch: 0.36 ms against 0.80 ms for the gather form, the same
+// candidates; writing the bounds costs the spectrum pass about 0.1 ms.
 
 // Harmonic h (0-based, in the reference's addend order, fold 2 taking 3/4
 // before 1/4) is j/2^k with k = hK(h), j = hJ(h); fold k ends at h = 2^k - 2.
@@ -486,17 +601,6 @@ __host__ __device__ constexpr int run_offset(int h) {
 static_assert(run_chunks(30) <= 64 && (int)kBoundWaveBins / (int)kBoundPool / kBoundChunk + 1 <= 64,
               "one 64-lane instruction per run");
 
-// Rounds x up to the next half-precision value (an upper bound of x).
-__device__ inline _Float16 half_up(float x) {
-  _Float16 h = (_Float16)x;
-  if ((float)h < x) {
-    uint16_t b = __builtin_bit_cast(uint16_t, h);
-    b = (b == 0x8000u) ? (uint16_t)0x0001u : (b & 0x8000u) ? (uint16_t)(b - 1u) : (uint16_t)(b + 1u);
-    h = __builtin_bit_cast(_Float16, b);
-  }
-  return h;
-}
-
 typedef float psx_f4v __attribute__((ext_vector_type(4)));
 
 // The bound spectrum Q of every trial row, and the fundamental's threshold
@@ -504,12 +608,12 @@ typedef float psx_f4v __attribute__((ext_vector_type(4)));
 // src/kernels.cu:391-423, peakfinder.hpp:78-83) on the same read of P.
 // Each lane takes four bins (one 16-byte load) and writes two entries.
 __global__ void __launch_bounds__(kB) k_pool_peaks(const float* __restrict__ P, int64_t pstride, uint32_t nb,
-                                                   Windows w, _Float16* __restrict__ Q, int64_t qstride,
+                                                   Windows w, uint16_t* __restrict__ Q, int64_t qstride,
                                                    const TrialDesc* __restrict__ descs, Cand16* __restrict__ out,
                                                    uint32_t cap, uint32_t* __restrict__ count) {
   const uint32_t tr = blockIdx.y;
   const float* p = P + (int64_t)tr * pstride;
-  _Float16* q = Q + (int64_t)tr * qstride;
+  uint16_t* q = Q + (int64_t)tr * qstride;
   const unsigned lane = threadIdx.x & 63u;
   const uint32_t start = (uint32_t)max(w.start[0], 0), end = (uint32_t)max(w.end[0], 0);
   const uint32_t ngroups = (nb + 3) / 4;
@@ -526,9 +630,7 @@ __global__ void __launch_bounds__(kB) k_pool_peaks(const float* __restrict__ P, 
       x[1] = k0 + 1 < nb ? v.y : -INFINITY;
       x[2] = k0 + 2 < nb ? v.z : -INFINITY;
       x[3] = k0 + 3 < nb ? v.w : -INFINITY;
-      const _Float16 q0 = half_up(fmaxf(x[0], x[1]));
-      const _Float16 q1 = half_up(fmaxf(x[2], x[3]));
-      const uint32_t pair = (uint32_t)__builtin_bit_cast(uint16_t, q0) | ((uint32_t)__builtin_bit_cast(uint16_t, q1) << 16);
+      const uint32_t pair = (uint32_t)bf16_up(fmaxf(x[0], x[1])) | ((uint32_t)bf16_up(fmaxf(x[2], x[3])) << 16);
       *(PSX_GLOBAL uint32_t*)(q + 2 * g) = pair;
     }
     bool any = false;
@@ -559,7 +661,7 @@ struct FoldThr {
 
 // Copies one run of Q (entries s0..s1, rounded out to whole chunks) into
 // the wave's LDS at dst; returns the first entry copied.
-__device__ inline uint32_t stage_run(const PSX_GLOBAL _Float16* q, uint32_t s0, uint32_t s1, _Float16* dst,
+__device__ inline uint32_t stage_run(const PSX_GLOBAL uint16_t* q, uint32_t s0, uint32_t s1, uint16_t* dst,
                                      unsigned lane) {
   const uint32_t a0 = s0 & ~(uint32_t)(kBoundChunk - 1);
   const uint32_t n = (s1 - a0) / kBoundChunk + 1;
@@ -571,7 +673,7 @@ __device__ inline uint32_t stage_run(const PSX_GLOBAL _Float16* q, uint32_t s0, 
 
 template <int NH, int H>
 struct BoundRuns {
-  __device__ static void stage(const PSX_GLOBAL _Float16* q, uint32_t base, _Float16* my, uint32_t (&a0)[NH + 1],
+  __device__ static void stage(const PSX_GLOBAL uint16_t* q, uint32_t base, uint16_t* my, uint32_t (&a0)[NH + 1],
                                unsigned lane) {
     if constexpr (H < NH) {
       constexpr int K = hK(H), J = hJ(H);
@@ -583,16 +685,16 @@ struct BoundRuns {
   }
   // adds harmonic H's bound to every running sum; at a fold's last addend
   // marks the bins whose bound crosses that fold's threshold
-  __device__ static void add(const _Float16* my, const uint32_t (&a0)[NH + 1], uint32_t i0, const FoldThr& ft,
+  __device__ static void add(const uint16_t* my, const uint32_t (&a0)[NH + 1], uint32_t i0, const FoldThr& ft,
                              float (&val)[kBoundD], bool (&fail)[kBoundD]) {
     if constexpr (H < NH) {
       constexpr int K = hK(H), J = hJ(H);
       constexpr uint32_t c = 1u << (K - 1);
       constexpr uint32_t step = (64u * J) >> K;  // source step between a lane's bins
       static_assert(step % kBoundPool == 0, "constant LDS offsets per d");
-      const _Float16* b = my + run_offset(H) + (((J * i0 + c) >> K) / kBoundPool - a0[H]);
+      const uint16_t* b = my + run_offset(H) + (((J * i0 + c) >> K) / kBoundPool - a0[H]);
 #pragma unroll
-      for (int d = 0; d < kBoundD; ++d) val[d] += (float)b[d * (step / kBoundPool)];
+      for (int d = 0; d < kBoundD; ++d) val[d] += bf16_value(b[d * (step / kBoundPool)]);
       if constexpr (((H + 2) & (H + 1)) == 0) {  // H = 2^K - 2: fold K complete
 #pragma unroll
         for (int d = 0; d < kBoundD; ++d) fail[d] |= val[d] > ft.t[K];
@@ -604,13 +706,13 @@ struct BoundRuns {
 
 template <int NF>
 __global__ void __launch_bounds__(kB) k_harm_bound(const float* __restrict__ P, int64_t pstride,
-                                                   const _Float16* __restrict__ Q, int64_t qstride, Windows w,
+                                                   const uint16_t* __restrict__ Q, int64_t qstride, Windows w,
                                                    FoldThr ft, const TrialDesc* __restrict__ descs,
                                                    uint32_t xcd_order, Cand16* __restrict__ out, uint32_t cap,
                                                    uint32_t* __restrict__ count) {
   constexpr int NH = (1 << NF) - 1;
   constexpr int WE = run_offset(NH);  // LDS entries per wave
-  __shared__ _Float16 lds[WE * (kB / 64)];
+  __shared__ uint16_t lds[WE * (kB / 64)];
   uint32_t tr = blockIdx.y, bx = blockIdx.x;
   if (xcd_order) {
     // workgroups reach the 8 XCDs round robin: hardware block b runs on XCD
@@ -623,10 +725,10 @@ __global__ void __launch_bounds__(kB) k_harm_bound(const float* __restrict__ P, 
     bx = k % gridDim.x;
   }
   const float* p = P + (int64_t)tr * pstride;
-  const PSX_GLOBAL _Float16* qg = (const PSX_GLOBAL _Float16*)(Q + (int64_t)tr * qstride);
+  const PSX_GLOBAL uint16_t* qg = (const PSX_GLOBAL uint16_t*)(Q + (int64_t)tr * qstride);
   const unsigned lane = threadIdx.x & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  _Float16* my = lds + wave * WE;
+  uint16_t* my = lds + wave * WE;
   const uint32_t hi = (uint32_t)w.hi;
   const uint32_t wstep = gridDim.x * (kB / 64) * kBoundWaveBins;
   for (uint32_t base = (uint32_t)w.lo + (bx * (kB / 64) + wave) * kBoundWaveBins; base < hi; base += wstep) {
@@ -640,10 +742,10 @@ __global__ void __launch_bounds__(kB) k_harm_bound(const float* __restrict__ P, 
     float val[kBoundD];
     bool fail[kBoundD];
     {
-      const _Float16* b = my + (i0 / kBoundPool - a0[NH]);
+      const uint16_t* b = my + (i0 / kBoundPool - a0[NH]);
 #pragma unroll
       for (int d = 0; d < kBoundD; ++d) {
-        val[d] = (float)b[d * (64 / kBoundPool)];
+        val[d] = bf16_value(b[d * (64 / kBoundPool)]);
         fail[d] = false;
       }
     }
@@ -728,17 +830,29 @@ int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, con
 }
 
 int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t pstride, const RootTable& rt,
-                     const TrialDesc* d_descs, uint32_t ntr, hipStream_t s) {
+                     const TrialDesc* d_descs, uint32_t ntr, hipStream_t s, const SpecBound* bound) {
   if (R < 4 || R % 2 || L % kColTile || (uint64_t)R * L > 0x7FFFFFFFull || pstride < (int64_t)R * L + 1)
     return psx::fail(PSX_ERR_ARG, "spec_rows: unsupported layout %u x %u", R, L);
+  // the bound pairs of a tile's rows need an even tile base row minus one
+  if (bound && (R % 4 || bound->qstride < bound_stride(pstride)))
+    return psx::fail(PSX_ERR_ARG, "spec_rows: unsupported bound layout %u x %u", R, L);
   // row tiles of rows [1, R/2), then one row of workgroups for the edge bins
   const unsigned row_tiles = (R / 2 - 1 + kRowTile - 1) / kRowTile;
   static_assert(kColTile * 2 <= kB, "the edge row needs 2L+1 threads from L/kColTile workgroups");
   const dim3 grid(L / kColTile, row_tiles + 1, ntr);
-  if ((R & (R - 1)) == 0 && R <= (1u << rt.shift))
-    k_spec_rows<true><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+  BoundOut bo{};
+  if (bound)
+    bo = BoundOut{bound->Q, bound->qstride, max(bound->w.start[0], 0), max(bound->w.end[0], 0),
+                  bound->w.thr, bound->out, bound->cap, bound->count};
+  const bool hi_fixed = (R & (R - 1)) == 0 && R <= (1u << rt.shift);
+  if (hi_fixed && bound)
+    k_spec_rows<true, true><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
+  else if (hi_fixed)
+    k_spec_rows<true, false><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
+  else if (bound)
+    k_spec_rows<false, true><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
   else
-    k_spec_rows<false><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs);
+    k_spec_rows<false, false><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
   PSX_CHECK_LAUNCH("k_spec_rows");
   return PSX_OK;
 }
@@ -778,8 +892,7 @@ int launch_pool_peaks(const float* P, int64_t pstride, uint64_t nb, const Window
   if (nb < 1 || nb > (1ull << 27) || (int64_t)nb > pstride || pstride % 4 || qstride < bound_stride(pstride))
     return psx::fail(PSX_ERR_ARG, "pool_peaks: unsupported row layout");
   const unsigned gx = psx::grid_for((nb + 3) / 4, kB, 1024);
-  k_pool_peaks<<<dim3(gx, ntr), kB, 0, s>>>(P, pstride, (uint32_t)nb, w, (_Float16*)Q, qstride, d_descs, out, cap,
-                                            count);
+  k_pool_peaks<<<dim3(gx, ntr), kB, 0, s>>>(P, pstride, (uint32_t)nb, w, Q, qstride, d_descs, out, cap, count);
   PSX_CHECK_LAUNCH("k_pool_peaks");
   return PSX_OK;
 }
@@ -800,7 +913,7 @@ int launch_harm_bound(const float* P, int64_t pstride, const uint16_t* Q, int64_
   const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), (uint64_t)kBoundWaveBins * (kB / 64), 2048);
   const uint32_t xcd = ntr % 8 == 0 ? 1u : 0u;
   const dim3 g(gx, ntr);
-  const _Float16* q = (const _Float16*)Q;
+  const uint16_t* q = Q;
   switch (nfolds) {
     case 1: k_harm_bound<1><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
     case 2: k_harm_bound<2><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;

@@ -29,10 +29,24 @@ struct Windows {
 
 int launch_spec_post(const float2* Z, float* P, uint64_t M, int64_t pstride, const psx::fft::RootTable& rt,
                      const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s);
+// What the row-layout spectrum pass also produces for the bound-filtered
+// harmonic pass: Q (see launch_pool_peaks) and the fundamental's
+// above-threshold bins, appended to out.
+struct SpecBound {
+  uint16_t* Q;
+  int64_t qstride;
+  Windows w;
+  Cand16* out;
+  uint32_t cap;
+  uint32_t* count;
+};
 // Spectrum pass over the transposed layout of a row-writing last FFT pass
 // (row r of L elements holds Z[r + R*e]); same output as launch_spec_post.
+// bound (optional): also writes Q and the fundamental's candidates, which
+// launch_pool_peaks would otherwise compute from P.
 int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t pstride,
-                     const psx::fft::RootTable& rt, const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s);
+                     const psx::fft::RootTable& rt, const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s,
+                     const SpecBound* bound = nullptr);
 // Bins per wave of the harmonic pass (8 per lane: measured 9% faster than
 // 16 and 7% faster than 4 at 2^23).  It reads up to kHarmWaveBins - 1
 // floats past the window end of a spectrum row, so a spectrum buffer of B
@@ -42,7 +56,7 @@ int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_
                       const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count,
                       hipStream_t s);
 // Bound-filtered harmonic pass.  Q holds, per trial row of qstride 16-bit
-// entries, half-precision upper bounds of bin pairs (entry u >= P[2u] and
+// entries, bfloat16 upper bounds of bin pairs (entry u >= P[2u] and
 // P[2u+1], rounded up); launch_pool_peaks writes it from P together with
 // the fundamental's above-threshold bins, launch_harm_bound appends the
 // folds' (nh >= 1) from Q, summing P exactly only where a bound crosses.

@@ -73,7 +73,7 @@ Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t s
     // them (read only for bins past the window) holds -inf
     qstride = bound_stride(pstride);
     dalloc(&Q, (uint64_t)qstride * B);
-    hip_ok(hipMemsetD16(Q, 0xFC00u, (size_t)qstride * B), "bound spectrum fill");
+    hip_ok(hipMemsetD16(Q, 0xFF80u, (size_t)qstride * B), "bound spectrum fill");  // bfloat16 -inf
   }
   for (Slot& sl : slots) {
     dalloc(&sl.d_desc, B);
@@ -169,20 +169,82 @@ void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, floa
 // Spectrum pass: from the natural half-length spectra, or from the
 // transposed rows when the plan's last pass writes those (its 32/64-byte
 // column runs cost more than the spectrum pass's tiled transpose).
+// In the bound-filtered form the row-layout pass also writes the bound
+// spectrum and the fundamental's candidates; after the natural-layout pass a
+// pooling pass computes them from P.
 void Engine::spectrum(Slot& sl, uint32_t n) {
-  if (rows_out)
-    psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s), "spec_rows");
-  else
+  // the row-layout pass pairs bins (2u, 2u+1) by rows, which needs R % 4 == 0
+  // (true of every plan with that layout: R is 2^k or 3*2^k, k >= 5)
+  const bool fused = bound && rows_out && rows_R % 4 == 0;
+  if (rows_out) {
+    const SpecBound sb{Q, qstride, win, sl.d_cands, cap, sl.d_count};
+    psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s, fused ? &sb : nullptr),
+           "spec_rows");
+  } else {
     psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
-}
-
-// Harmonic pass: the bound-filtered form (pooling pass with the
-// fundamental's threshold, then the folds from the bounds) or the exact
-// gathers of every addend.
-void Engine::harmonics(Slot& sl, uint32_t n) {
-  if (bound) {
+  }
+  if (bound && !fused)
     psx_ok(launch_pool_peaks(P, pstride, nb, win, Q, qstride, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
            "pool_peaks");
+}
+
+namespace {
+// bfloat16 bits of the smallest bfloat16 >= x (x not NaN), computed as the
+// nearest-up rounding of the float's top half: exact for every float
+uint16_t bf16_up_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if (u & 0x80000000u) return (uint16_t)(u >> 16);  // negative: toward zero is up
+  const float t = [&] {
+    const uint32_t top = u & 0xFFFF0000u;
+    float f;
+    std::memcpy(&f, &top, 4);
+    return f;
+  }();
+  if (t == x) return (uint16_t)(u >> 16);
+  return (uint16_t)((u >> 16) + 1u);  // the next bfloat16 up (+inf past the largest finite)
+}
+}  // namespace
+
+void Engine::audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out) {
+  if (!bound) throw std::runtime_error("audit_bounds: the engine runs the exact harmonic pass");
+  if (n < 1 || n > B) throw std::runtime_error("audit_bounds: 1..B trials");
+  if (!pending.empty() || !ready.empty()) throw std::runtime_error("audit_bounds: engine busy");
+  Slot& sl = slots[0];
+  fill_slot(sl, trials, n);
+  hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s), "desc");
+  psx_ok(launch_zero(sl.d_count, s), "zero");
+  psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
+  spectrum(sl, n);
+  std::vector<float> hp((size_t)pstride * n);
+  std::vector<uint16_t> hq((size_t)qstride * n);
+  uint32_t count = 0;
+  hip_ok(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(float), hipMemcpyDeviceToHost, s), "P copy");
+  hip_ok(hipMemcpyAsync(hq.data(), Q, hq.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, s), "Q copy");
+  hip_ok(hipMemcpyAsync(&count, sl.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count copy");
+  hip_ok(hipStreamSynchronize(s), "audit sync");
+  out = BoundAudit{};
+  const uint64_t nq = (nb + 1) / 2;
+  for (uint32_t t = 0; t < n; ++t) {
+    const float* pr = hp.data() + (size_t)t * pstride;
+    const uint16_t* qr = hq.data() + (size_t)t * qstride;
+    for (uint64_t u = 0; u < nq; ++u) {
+      const float a = pr[2 * u];
+      const float m = 2 * u + 1 < nb ? std::max(a, pr[2 * u + 1]) : a;
+      out.entries++;
+      if (qr[u] != bf16_up_bits(m)) out.mismatches++;
+    }
+    for (uint64_t k = (uint64_t)std::max(win.start[0], 0); k < (uint64_t)std::max(win.end[0], 0); ++k)
+      if (pr[k] > win.thr) out.fundamentals++;
+  }
+  out.appended = count;
+}
+
+
+// Harmonic pass: the bound-filtered form (the folds from the bounds the
+// spectrum stage wrote) or the exact gathers of every addend.
+void Engine::harmonics(Slot& sl, uint32_t n) {
+  if (bound) {
     psx_ok(launch_harm_bound(P, pstride, Q, qstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
            "harm_bound");
   } else {
@@ -435,14 +497,8 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
       psx_ok(psx::fft::exec_fused_pass(plan, k, sl.d_desc, n, Z, s, rows_out), "fused pass");
     } else if (k == np) {
       spectrum(sl, n);
-      if (bound)
-        psx_ok(launch_pool_peaks(P, pstride, nb, win, Q, qstride, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
-               "pool_peaks");
-    } else if (bound) {
-      psx_ok(launch_harm_bound(P, pstride, Q, qstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
-             "harm_bound");
     } else {
-      psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
+      harmonics(sl, n);
     }
   };
   for (int r = 0; r < reps + 1; ++r) {

@@ -49,6 +49,14 @@ struct EngineOptions {
   int harmonic_pass = PSX_HARMONIC_BOUND;
 };
 
+// Self-check of the bound spectrum one batch leaves (Engine::audit_bounds).
+struct BoundAudit {
+  uint64_t entries = 0;       // bound entries checked (all trials)
+  uint64_t mismatches = 0;    // entries differing from the bound recomputed on the host from P
+  uint64_t fundamentals = 0;  // bins above threshold in the fundamental's window (host scan of P)
+  uint64_t appended = 0;      // candidates the device appended (the fundamental's, before the harmonic pass)
+};
+
 struct StageTimes {
   // per launch: FFT passes (first one with resampling), spectrum pass (with
   // the bound pooling and the fundamental's threshold in the bound form),
@@ -112,6 +120,9 @@ class Engine {
   // Average duration of each launch of one batch of n trials, timed with HIP
   // events on the engine stream over `reps` repetitions (after one warm-up).
   void profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& out);
+  // Runs the FFT and spectrum stages of one batch of n trials and checks the
+  // bound spectrum against P on the host (bound form only).
+  void audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out);
 
   // Device bytes held by an engine of this shape (for admission control).
   static uint64_t device_bytes(uint64_t N, uint32_t B, uint32_t cap);
@@ -151,7 +162,7 @@ class Engine {
   psx::fft::RootTable rt{};
   float2* Z = nullptr;
   float* P = nullptr;
-  // bound-filtered harmonic pass: Q rows of qstride half-precision bounds
+  // bound-filtered harmonic pass: Q rows of qstride bfloat16 bounds
   bool bound = true;
   int64_t qstride = 0;
   uint16_t* Q = nullptr;

@@ -246,6 +246,42 @@ extern "C" int psx_batch_profile(const psx_search_args* args, const psx_batch_op
   }
 }
 
+extern "C" int psx_batch_bound_audit(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
+                                     uint64_t nsamps, const float* accs, uint32_t naccs, uint64_t* result) {
+  int rc = psx::search::validate_args(args);
+  if (rc) return rc;
+  PSX_REQUIRE(d_series && accs && naccs > 0 && result, "psx_batch_bound_audit: bad arguments");
+  hipStream_t s = nullptr;
+  float* d_tim = nullptr;
+  try {
+    EngineOptions eo;
+    psx::fft::Decomp forced;
+    engine_options(opts, eo, forced);
+    PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
+    PSX_CHECK_HIP(hipMalloc(&d_tim, sizeof(float) * args->size));
+    float mean, sd;
+    eng->prologue(d_series, nsamps, d_tim, &mean, &sd, nullptr);
+    const uint32_t n = std::min(naccs, eng->batch());
+    std::vector<TrialWork> tw(n);
+    for (uint32_t t = 0; t < n; ++t) tw[t] = TrialWork{d_tim, accs[t], mean, sd, 0.0f, 0};
+    psx::batch::BoundAudit a;
+    eng->audit_bounds(tw.data(), n, a);
+    result[0] = a.entries;
+    result[1] = a.mismatches;
+    result[2] = a.fundamentals;
+    result[3] = a.appended;
+    eng.reset();
+    (void)hipFree(d_tim);
+    (void)hipStreamDestroy(s);
+    return psx::ok();
+  } catch (const std::exception& e) {
+    if (d_tim) (void)hipFree(d_tim);
+    if (s) (void)hipStreamDestroy(s);
+    return psx::fail(PSX_ERR_HIP, "psx_batch_bound_audit: %s", e.what());
+  }
+}
+
 extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opts* opts, const float* series,
                                 int series_on_device, uint64_t nsamps, const float* dm_list, uint32_t ndm,
                                 const float* accs, uint32_t naccs, psx_candidate* out, uint32_t cap, uint32_t* count,

@@ -29,13 +29,16 @@ def test_stage_bytes_model():
     assert three[:3] == [4 * n + 8 * m, 16 * m, 16 * m] and three[3:] == two[2:]
     # the fused trial moves 24n bytes with two FFT passes (about 28n with three)
     assert abs(sum(two) - 24 * n) <= 16
-    # bound-filtered harmonic form: the pooling pass reads the spectrum again
-    # and writes one 16-bit bound per two bins (n/2 + 2 bytes), which the
-    # harmonic pass reads once
+    # bound-filtered harmonic form: one 16-bit bound per two bins (n/2 + 2
+    # bytes) written with the spectrum (by a pooling pass that reads the
+    # spectrum again after the natural layout), read once by the harmonic pass
     qb = nb + 1
-    bound = bench.stage_bytes(n, 2)
+    bound = bench.stage_bytes(n, 2, fused=False)
     assert bound[:2] == two[:2]
     assert bound[2] == 8 * m + 8 * nb + qb and bound[3] == qb
+    # fused into the row-layout spectrum pass: no second read of the spectrum
+    fused = bench.stage_bytes(n, 2)
+    assert fused[2] == 8 * m + 4 * nb + qb and fused[3] == qb
 
 
 def test_cpu_baseline_threads_and_model():

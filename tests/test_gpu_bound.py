@@ -1,5 +1,5 @@
 """The bound-filtered harmonic pass returns exactly the candidates of the
-exact-gather pass (and of the plain Worker): it sums half-precision upper
+exact-gather pass (and of the plain Worker): it sums bfloat16 upper
 bounds of bin pairs first and sums the spectrum exactly only where a bound
 crosses the threshold.  Cases: every fold count, thresholds low enough that
 most bins cross (bounds and exact values both), trial counts that do and do
@@ -87,3 +87,41 @@ def test_bound_headline_length_matches_worker(psx):
     print("2^23: %d candidates" % len(ref), flush=True)
     assert same(got, ref) and len(got) == len(ref)
     assert np.any((got["dm_idx"] == 1) & (got["snr"] > 10))
+
+
+@pytest.mark.parametrize("n,decomp,layout", [
+    (1 << 17, None, "auto"),
+    (1 << 20, [(1024, 8), (512, 16)], "auto"),
+    (1 << 21, [(1024, 8), (1024, 4)], "rows"),
+    (1 << 22, [(1024, 8), (2048, 4)], "rows"),
+    (3 << 21, [(2048, 8), (1536, 4)], "rows"),
+    (3 << 21, [(2048, 8), (1536, 4)], "natural"),
+    (1 << 23, [(1024, 16), (4096, 4)], "rows"),
+    (1 << 23, [(2048, 8), (2048, 4)], "rows"),
+])
+def test_bound_spectrum_audit(psx, n, decomp, layout):
+    """Every bound entry the spectrum stage leaves equals the bfloat16
+    value rounded up from max(P[2u], P[2u+1]) recomputed on the host from the
+    same spectra (row-layout tiles, their edge rows and bin n/2, and the
+    pooling pass after the natural layout), and the fundamental's appended
+    candidates are exactly the bins above threshold in its window."""
+    import ctypes as C
+    import peasoup_amd.workloads as W
+    lay = {"auto": W.SPECTRUM_AUTO, "rows": W.SPECTRUM_ROWS, "natural": W.SPECTRUM_NATURAL}[layout]
+    w = W.Workload("aud", 1, n, 64e-6, 0x5EB0D0, 5.0, -50.0, 50.0, min_snr=3.0,
+                   injections=[W.Injection(0, 0.0071, 20.0, 25.0, 0.2)])
+    d = W.generate_device(w)
+    accs = np.ascontiguousarray(W.accel_list(n, w.tsamp, -50.0, 50.0)[:8], np.float32)
+    psx.register_signatures({"psx_batch_bound_audit": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
+                                                                 C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
+                                                                 C.c_void_p])})
+    res = np.zeros(4, np.uint64)
+    opts = W.make_batch_opts(8, decomp=decomp, spectrum_layout=lay)
+    psx.check(psx.lib().psx_batch_bound_audit(C.byref(w.search_args()), C.byref(opts), d.ptr, n,
+                                              accs.ctypes.data, accs.size, res.ctypes.data))
+    entries, mismatches, fundamentals, appended = (int(x) for x in res)
+    print("n %d %s %s: %d entries, %d mismatches, %d fundamentals, %d appended"
+          % (n, decomp, layout, entries, mismatches, fundamentals, appended), flush=True)
+    assert entries == accs.size * ((n // 2 + 2) // 2)
+    assert mismatches == 0
+    assert fundamentals > 0 and appended == fundamentals
