@@ -320,8 +320,10 @@ int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, c
  * from max(P[2u], P[2u+1])) and the fundamental's above-threshold bins.
  * result[0] = entries checked, [1] = entries that differ (0 when correct),
  * [2] = fundamental bins above threshold on the host, [3] = candidates the
- * device appended (equal to [2] when correct).  For tests and for validating
- * a new FFT decomposition. */
+ * device appended (equal to [2] when correct), [4] = bins of the harmonic
+ * window whose bound sums cross at some fold (the bins summed exactly),
+ * [5] = bins of the window (result holds 6 values).  For tests and for
+ * validating a new FFT decomposition. */
 int psx_batch_bound_audit(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
                           uint64_t nsamps, const float* accs, uint32_t naccs, uint64_t* result);
 

@@ -181,22 +181,25 @@ __device__ inline void untangle_pair(float2 zk, float2 zm, uint32_t k, uint32_t 
 
 // Bins of rows 0 and R/2 (k = R*e and R/2 + R*e) and the Nyquist bin M of
 // the transposed layout, one thread each, through the natural-index view.
-__device__ inline float2 z_rows(const float2* y, uint32_t R, uint32_t L, uint32_t j) {
+// rshift: log2(R) when R is a power of two (the row and column of j by a
+// mask and a shift), else 32 (by division).
+__device__ inline float2 z_rows(const float2* y, uint32_t R, uint32_t L, uint32_t j, uint32_t rshift = 32) {
   const uint32_t M = R * L;
   if (j >= M) j -= M;
-  return y[(j % R) * L + j / R];
+  return rshift < 32 ? y[(j & (R - 1)) * L + (j >> rshift)] : y[(j % R) * L + j / R];
 }
-__device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_t k, const RootTable& rt) {
+__device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_t k, const RootTable& rt,
+                                uint32_t rshift = 32) {
   const uint32_t M = R * L;
   float2 xk, xm;
-  untangle_pair(z_rows(y, R, L, k), z_rows(y, R, L, M - k), k, M, rt, xk, xm);
+  untangle_pair(z_rows(y, R, L, k, rshift), z_rows(y, R, L, M - k, rshift), k, M, rt, xk, xm);
   return xk;
 }
 // t < L: row 0; t < 2L: row R/2; t == 2L: bin M.  Returns the bin and its
 // value; with bo, also the bound of the bin pair (k, k+1), whose second bin
 // (row 1 or R/2 + 1, a tile's) is recomputed here through the same untangle
 // and interbin arithmetic, so it equals the tile's value bit for bit.
-template <bool BOUND>
+template <bool BOUND, bool WRITE_P>
 __device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uint32_t L, uint32_t t,
                                       const RootTable& rt, float mean_n, float std_n, uint16_t* q, uint32_t& k) {
   const uint32_t M = R * L;
@@ -211,7 +214,7 @@ __device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uin
   }
   if (k > 0) prev = x_rows(y, R, L, k - 1, rt);
   const float v = normalised(x, prev, mean_n, std_n);
-  p[k] = v;
+  if constexpr (WRITE_P) p[k] = v;
   if constexpr (BOUND) {
     float m = v;  // bin M has no partner (n/2 + 1 bins)
     if (k < M) m = fmaxf(v, normalised(x_rows(y, R, L, k + 1, rt), x, mean_n, std_n));
@@ -220,7 +223,10 @@ __device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uin
   return v;
 }
 
-template <bool HI_FIXED, bool BOUND>
+// BOUND: also the bound spectrum and the fundamental's candidates; WRITE_P:
+// the spectra themselves (the bound-filtered harmonic pass recomputes the
+// few bins it sums exactly from Y instead).
+template <bool HI_FIXED, bool BOUND, bool WRITE_P>
 __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
                                                   uint32_t L, int64_t pstride, RootTable rt,
                                                   const TrialDesc* __restrict__ descs, BoundOut bo) {
@@ -240,7 +246,7 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
     const bool valid = t <= 2 * L;
     uint32_t k = 0;
     float v = 0.f;
-    if (valid) v = spec_edge_bin<BOUND>(y, p, R, L, t, rt, mean_n, std_n, q, k);
+    if (valid) v = spec_edge_bin<BOUND, WRITE_P>(y, p, R, L, t, rt, mean_n, std_n, q, k);
     if constexpr (BOUND) append_fundamental(valid && fundamental_hit(v, k, bo), descs[tr].slot, k, v, bo);
     return;
   }
@@ -370,8 +376,10 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
       const uint32_t ku = (R - r) + R * (L - 1 - (ea + j));
       const float vl = lo_t[j][rr], vu = up_t[j][rr];
       if (rv) {
-        stg1(p, kl, vl);
-        stg1(p, ku, vu);
+        if constexpr (WRITE_P) {
+          stg1(p, kl, vl);
+          stg1(p, ku, vu);
+        }
         any |= fmaxf(vl, vu) > bo.thr;
       }
       if (pv) {
@@ -704,10 +712,47 @@ struct BoundRuns {
   }
 };
 
-template <int NF>
-__global__ void __launch_bounds__(kB) k_harm_bound(const float* __restrict__ P, int64_t pstride,
+// Where the exact sums of the bins whose bound crosses take the spectrum
+// from: P, or (rows != nullptr) bin by bin from the row-layout FFT output the
+// spectrum pass read, through its untangle, interbin and normalisation
+// (x_rows and normalised: for every bin the same operations on the same
+// inputs as the spectrum pass, so the same value bit for bit).
+struct ExactSrc {
+  const float2* rows;  // Y of trial 0 (trial stride R*L), or nullptr: read P
+  uint32_t R, L;
+  uint32_t rshift;     // log2(R) for a power of two, else 32
+  RootTable rt;
+};
+// harmonic_sums32's sums (the same addends in the same order) with the
+// spectrum value of bin x given by f(x), one addend per iteration of a loop
+// the compiler keeps rolled: this path runs for the rare bins whose bound
+// crosses, and an unrolled recomputation of 2^NF spectrum values would hold
+// enough registers to cost the main loop its occupancy.
+template <int NF, class F>
+__device__ inline void harmonic_sums_rolled(F f, uint32_t i, float (&s)[5]) {
+  float val = f(i);
+#pragma unroll 1
+  for (int h = 0; h < (1 << NF) - 1; ++h) {
+    const int K = hK(h), J = hJ(h);
+    val += f(((uint32_t)J * i + (1u << (K - 1))) >> K);
+    if (h == 0) s[0] = psx::fold_scale<1>(val);
+    else if (h == 2) s[1] = psx::fold_scale<2>(val);
+    else if (h == 6) s[2] = psx::fold_scale<3>(val);
+    else if (h == 14) s[3] = psx::fold_scale<4>(val);
+    else if (h == 30) s[4] = psx::fold_scale<5>(val);
+  }
+}
+
+// Register budget: eight waves per SIMD (the LDS runs allow eight
+// four-wave workgroups per CU) up to four folds; the rare exact path's
+// scalar registers would otherwise take one of them.
+constexpr int bound_waves(int nf) { return nf <= 4 ? 8 : 3; }
+
+template <int NF, bool FROM_ROWS>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(bound_waves(NF)))) k_harm_bound(const float* __restrict__ P, int64_t pstride,
                                                    const uint16_t* __restrict__ Q, int64_t qstride, Windows w,
                                                    FoldThr ft, const TrialDesc* __restrict__ descs,
+                                                   const ExactSrc* __restrict__ srcp,
                                                    uint32_t xcd_order, Cand16* __restrict__ out, uint32_t cap,
                                                    uint32_t* __restrict__ count) {
   constexpr int NH = (1 << NF) - 1;
@@ -763,7 +808,24 @@ __global__ void __launch_bounds__(kB) k_harm_bound(const float* __restrict__ P, 
       const uint32_t i = i0 + 64u * (uint32_t)d;
       const bool valid = fail[d] && i < hi;
       float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-      if (valid) psx::harmonic_sums32<NF>(p, i, s);
+      if (valid) {
+        if constexpr (FROM_ROWS) {
+          // read from memory here, not held in scalar registers by the loop
+          const ExactSrc src = *srcp;
+          const uint32_t M = src.R * src.L;
+          const float2* y = src.rows + (uint64_t)tr * M;
+          const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
+          auto pval = [&](uint32_t x) {
+            const float2 xk = x_rows(y, src.R, src.L, x, src.rt, src.rshift);
+            const float2 prev =
+                x > 0 ? x_rows(y, src.R, src.L, x - 1, src.rt, src.rshift) : make_float2(0.f, 0.f);
+            return normalised(xk, prev, mean_n, std_n);
+          };
+          harmonic_sums_rolled<NF>(pval, i, s);
+        } else {
+          psx::harmonic_sums32<NF>(p, i, s);
+        }
+      }
       bool hit[NF];
       uint64_t mask[NF];
       uint32_t total = 0;
@@ -845,14 +907,16 @@ int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t 
     bo = BoundOut{bound->Q, bound->qstride, max(bound->w.start[0], 0), max(bound->w.end[0], 0),
                   bound->w.thr, bound->out, bound->cap, bound->count};
   const bool hi_fixed = (R & (R - 1)) == 0 && R <= (1u << rt.shift);
-  if (hi_fixed && bound)
-    k_spec_rows<true, true><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
-  else if (hi_fixed)
-    k_spec_rows<true, false><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
-  else if (bound)
-    k_spec_rows<false, true><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
+  if (!bound && !P) return psx::fail(PSX_ERR_ARG, "spec_rows: no spectrum buffer");
+  const bool write_p = P != nullptr;
+#define PSX_SPEC_ROWS(HF, BD, WP) k_spec_rows<HF, BD, WP><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo)
+  if (!bound)
+    hi_fixed ? PSX_SPEC_ROWS(true, false, true) : PSX_SPEC_ROWS(false, false, true);
+  else if (write_p)
+    hi_fixed ? PSX_SPEC_ROWS(true, true, true) : PSX_SPEC_ROWS(false, true, true);
   else
-    k_spec_rows<false, false><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo);
+    hi_fixed ? PSX_SPEC_ROWS(true, true, false) : PSX_SPEC_ROWS(false, true, false);
+#undef PSX_SPEC_ROWS
   PSX_CHECK_LAUNCH("k_spec_rows");
   return PSX_OK;
 }
@@ -880,6 +944,17 @@ int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_
 }
 
 
+float fold_threshold_of(int k, float thr) {
+  switch (k) {
+    case 1: return fold_threshold<1>(thr);
+    case 2: return fold_threshold<2>(thr);
+    case 3: return fold_threshold<3>(thr);
+    case 4: return fold_threshold<4>(thr);
+    case 5: return fold_threshold<5>(thr);
+    default: return INFINITY;
+  }
+}
+
 int64_t bound_stride(int64_t pstride) {
   // a wave's staging reads up to kBoundWaveBins - 1 bins past the window
   // end, plus a chunk of alignment: per-row slack covers both
@@ -897,31 +972,46 @@ int launch_pool_peaks(const float* P, int64_t pstride, uint64_t nb, const Window
   return PSX_OK;
 }
 
+int write_rows_source(const RowsSource& rs, void* d_src) {
+  static_assert(sizeof(ExactSrc) <= kRowsSourceBytes, "kRowsSourceBytes");
+  if (!d_src || !rs.Y || rs.R < 4 || (uint64_t)rs.R * rs.L > 0x7FFFFFFFull)
+    return psx::fail(PSX_ERR_ARG, "write_rows_source: bad layout");
+  const uint32_t R = rs.R;
+  const ExactSrc e{rs.Y, R, rs.L, (R & (R - 1)) == 0 ? (uint32_t)__builtin_ctz(R) : 32u, rs.rt};
+  PSX_CHECK_HIP(hipMemcpy(d_src, &e, sizeof(e), hipMemcpyHostToDevice));
+  return PSX_OK;
+}
+
 int launch_harm_bound(const float* P, int64_t pstride, const uint16_t* Q, int64_t qstride, const Windows& w,
                       uint32_t nfolds, const TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap,
-                      uint32_t* count, hipStream_t s) {
+                      uint32_t* count, hipStream_t s, const RowsSource* rows, const void* d_rows_src) {
   if (w.hi <= w.lo) return PSX_OK;
   if (w.hi > (1ll << 27)) return psx::fail(PSX_ERR_UNSUPPORTED, "harm_bound: more than 2^27 bins");
   if (w.lo < 0 || w.hi > pstride || qstride < bound_stride(pstride))
     return psx::fail(PSX_ERR_ARG, "harm_bound: window outside the spectrum row");
+  if (!rows && !P) return psx::fail(PSX_ERR_ARG, "harm_bound: no spectrum for the exact sums");
+  if (rows && (!d_rows_src || w.hi > (int64_t)rows->R * rows->L + 1 ||
+               (uint64_t)rows->R * rows->L > 0x7FFFFFFFull))
+    return psx::fail(PSX_ERR_ARG, "harm_bound: window outside the row layout");
+  const ExactSrc* src = rows ? (const ExactSrc*)d_rows_src : nullptr;
   FoldThr ft{};
-  ft.t[1] = fold_threshold<1>(w.thr);
-  ft.t[2] = fold_threshold<2>(w.thr);
-  ft.t[3] = fold_threshold<3>(w.thr);
-  ft.t[4] = fold_threshold<4>(w.thr);
-  ft.t[5] = fold_threshold<5>(w.thr);
+  for (int k = 1; k <= 5; ++k) ft.t[k] = fold_threshold_of(k, w.thr);
   const unsigned gx = psx::grid_for((uint64_t)(w.hi - w.lo), (uint64_t)kBoundWaveBins * (kB / 64), 2048);
   const uint32_t xcd = ntr % 8 == 0 ? 1u : 0u;
   const dim3 g(gx, ntr);
   const uint16_t* q = Q;
+#define PSX_HB(NF, FR) \
+  k_harm_bound<NF, FR><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, src, xcd, out, cap, count)
+  const bool fr = rows != nullptr;
   switch (nfolds) {
-    case 1: k_harm_bound<1><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
-    case 2: k_harm_bound<2><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
-    case 3: k_harm_bound<3><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
-    case 4: k_harm_bound<4><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
-    case 5: k_harm_bound<5><<<g, kB, 0, s>>>(P, pstride, q, qstride, w, ft, d_descs, xcd, out, cap, count); break;
+    case 1: fr ? PSX_HB(1, true) : PSX_HB(1, false); break;
+    case 2: fr ? PSX_HB(2, true) : PSX_HB(2, false); break;
+    case 3: fr ? PSX_HB(3, true) : PSX_HB(3, false); break;
+    case 4: fr ? PSX_HB(4, true) : PSX_HB(4, false); break;
+    case 5: fr ? PSX_HB(5, true) : PSX_HB(5, false); break;
     default: return psx::fail(PSX_ERR_ARG, "nfolds must be 1..5");
   }
+#undef PSX_HB
   PSX_CHECK_LAUNCH("k_harm_bound");
   return PSX_OK;
 }

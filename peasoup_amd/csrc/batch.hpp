@@ -43,7 +43,8 @@ struct SpecBound {
 // Spectrum pass over the transposed layout of a row-writing last FFT pass
 // (row r of L elements holds Z[r + R*e]); same output as launch_spec_post.
 // bound (optional): also writes Q and the fundamental's candidates, which
-// launch_pool_peaks would otherwise compute from P.
+// launch_pool_peaks would otherwise compute from P; with bound, P may be
+// nullptr (no spectra written).
 int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t pstride,
                      const psx::fft::RootTable& rt, const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s,
                      const SpecBound* bound = nullptr);
@@ -62,14 +63,31 @@ int launch_harm_peaks(const float* P, int64_t pstride, const Windows& w, uint32_
 // folds' (nh >= 1) from Q, summing P exactly only where a bound crosses.
 constexpr uint32_t kBoundPool = 2;         // bins per entry of Q
 constexpr uint32_t kBoundWaveBins = 512;   // bins per wave of k_harm_bound
+// The largest float v with fold_scale<k>(v) <= thr: fold k's running sum
+// crosses the threshold iff it exceeds this value.
+float fold_threshold_of(int k, float thr);
 // Row stride (entries) of Q for spectrum rows of pstride floats.
 int64_t bound_stride(int64_t pstride);
 int launch_pool_peaks(const float* P, int64_t pstride, uint64_t nb, const Windows& w, uint16_t* Q, int64_t qstride,
                       const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap, uint32_t* count,
                       hipStream_t s);
+// The row-layout FFT output Y (R rows of L, trial stride R*L) from which
+// launch_harm_bound recomputes the bins it sums exactly when the spectrum
+// pass wrote no P.
+struct RowsSource {
+  const float2* Y;
+  uint32_t R, L;
+  psx::fft::RootTable rt;
+};
+// The exact sums read Y through a copy of rows in device memory (d_src,
+// kRowsSourceBytes, written once by write_rows_source), which keeps those
+// parameters out of the scalar registers of the kernel's main loop.
+constexpr size_t kRowsSourceBytes = 64;
+int write_rows_source(const RowsSource& rs, void* d_src);
 int launch_harm_bound(const float* P, int64_t pstride, const uint16_t* Q, int64_t qstride, const Windows& w,
                       uint32_t nfolds, const psx::fft::TrialDesc* d_descs, uint32_t ntr, Cand16* out, uint32_t cap,
-                      uint32_t* count, hipStream_t s);
+                      uint32_t* count, hipStream_t s, const RowsSource* rows = nullptr,
+                      const void* d_rows_src = nullptr);
 int launch_zero(uint32_t* p, hipStream_t s);
 
 }  // namespace batch

@@ -66,14 +66,21 @@ Engine::Engine(const psx_search_args& a, const EngineOptions& opt, hipStream_t s
   if (opt.harmonic_pass != PSX_HARMONIC_BOUND && opt.harmonic_pass != PSX_HARMONIC_EXACT)
     throw std::runtime_error("harmonic_pass must be PSX_HARMONIC_BOUND or PSX_HARMONIC_EXACT");
   bound = opt.harmonic_pass == PSX_HARMONIC_BOUND && F > 0;
+  // the row-layout pass pairs bins (2u, 2u+1) by rows, which needs R % 4 == 0
+  // (true of every plan with that layout: R is 2^k or 3*2^k, k >= 5)
+  fused = bound && rows_out && rows_R % 4 == 0;
   dalloc(&Z, M * B);
-  dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
+  if (!fused) dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
   if (bound) {
     // the pooling pass writes each row's bounds up to bin nb; the slack past
     // them (read only for bins past the window) holds -inf
     qstride = bound_stride(pstride);
     dalloc(&Q, (uint64_t)qstride * B);
     hip_ok(hipMemsetD16(Q, 0xFF80u, (size_t)qstride * B), "bound spectrum fill");  // bfloat16 -inf
+    if (fused) {
+      dalloc((uint8_t**)&rows_src, kRowsSourceBytes);
+      psx_ok(write_rows_source(RowsSource{Z, rows_R, rows_L, rt}, rows_src), "rows source");
+    }
   }
   for (Slot& sl : slots) {
     dalloc(&sl.d_desc, B);
@@ -131,6 +138,7 @@ Engine::~Engine() {
   (void)hipFree(Z);
   (void)hipFree(P);
   (void)hipFree(Q);
+  (void)hipFree(rows_src);
   psx_fft_plan_destroy(plan);
   psx_fft_plan_destroy(p_r2c);
   psx_fft_plan_destroy(p_c2r);
@@ -172,13 +180,11 @@ void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, floa
 // In the bound-filtered form the row-layout pass also writes the bound
 // spectrum and the fundamental's candidates; after the natural-layout pass a
 // pooling pass computes them from P.
-void Engine::spectrum(Slot& sl, uint32_t n) {
-  // the row-layout pass pairs bins (2u, 2u+1) by rows, which needs R % 4 == 0
-  // (true of every plan with that layout: R is 2^k or 3*2^k, k >= 5)
-  const bool fused = bound && rows_out && rows_R % 4 == 0;
+void Engine::spectrum(Slot& sl, uint32_t n, bool write_p) {
   if (rows_out) {
     const SpecBound sb{Q, qstride, win, sl.d_cands, cap, sl.d_count};
-    psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s, fused ? &sb : nullptr),
+    psx_ok(launch_spec_rows(Z, write_p ? P : nullptr, rows_R, rows_L, pstride, rt, sl.d_desc, n, s,
+                            fused ? &sb : nullptr),
            "spec_rows");
   } else {
     psx_ok(launch_spec_post(Z, P, M, pstride, rt, sl.d_desc, n, s), "spec_post");
@@ -212,10 +218,13 @@ void Engine::audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out) 
   if (!pending.empty() || !ready.empty()) throw std::runtime_error("audit_bounds: engine busy");
   Slot& sl = slots[0];
   fill_slot(sl, trials, n);
+  // the fused form writes no spectra: the audit has them written beside the
+  // bounds, by the same pass
+  if (!P) dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
   hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s), "desc");
   psx_ok(launch_zero(sl.d_count, s), "zero");
   psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
-  spectrum(sl, n);
+  spectrum(sl, n, true);
   std::vector<float> hp((size_t)pstride * n);
   std::vector<uint16_t> hq((size_t)qstride * n);
   uint32_t count = 0;
@@ -236,6 +245,31 @@ void Engine::audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out) 
     }
     for (uint64_t k = (uint64_t)std::max(win.start[0], 0); k < (uint64_t)std::max(win.end[0], 0); ++k)
       if (pr[k] > win.thr) out.fundamentals++;
+    // the harmonic pass's bound sums, in its order and float arithmetic
+    float ft[6];
+    for (uint32_t k = 1; k <= F; ++k) ft[k] = fold_threshold_of((int)k, win.thr);
+    auto qv = [&](uint64_t x) {
+      const uint32_t bits = (uint32_t)qr[x / 2] << 16;
+      float f;
+      std::memcpy(&f, &bits, 4);
+      return f;
+    };
+    for (uint64_t i = (uint64_t)win.lo; i < (uint64_t)win.hi; ++i) {
+      float val = qv(i);
+      bool cross = false;
+      int h = 0;
+      for (uint32_t k = 1; k <= F; ++k) {
+        const uint64_t c = 1ull << (k - 1);
+        // fold k's odd multiples j/2^k, fold 2 taking 3/4 before 1/4
+        for (uint64_t jj = 0; jj < c; ++jj, ++h) {
+          const uint64_t j = (k == 2) ? (jj == 0 ? 3 : 1) : 2 * jj + 1;
+          val += qv((j * i + c) >> k);
+        }
+        cross |= val > ft[k];
+      }
+      out.window++;
+      if (cross) out.crossing++;
+    }
   }
   out.appended = count;
 }
@@ -245,7 +279,9 @@ void Engine::audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out) 
 // spectrum stage wrote) or the exact gathers of every addend.
 void Engine::harmonics(Slot& sl, uint32_t n) {
   if (bound) {
-    psx_ok(launch_harm_bound(P, pstride, Q, qstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s),
+    const RowsSource rs{Z, rows_R, rows_L, rt};
+    psx_ok(launch_harm_bound(fused ? nullptr : P, pstride, Q, qstride, win, F, sl.d_desc, n, sl.d_cands, cap,
+                             sl.d_count, s, fused ? &rs : nullptr, rows_src),
            "harm_bound");
   } else {
     psx_ok(launch_harm_peaks(P, pstride, win, F, sl.d_desc, n, sl.d_cands, cap, sl.d_count, s), "harm_peaks");
@@ -296,7 +332,7 @@ void Engine::launch(Slot& sl, uint32_t n, bool run) {
            "descriptor upload");
     psx_ok(launch_zero(sl.d_count, s), "zero");
     psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
-    spectrum(sl, n);
+    spectrum(sl, n, !fused);
     harmonics(sl, n);
     hip_ok(hipMemcpyAsync(sl.h_count, sl.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count copy");
     hip_ok(hipMemcpyAsync(sl.h_cands, sl.d_cands, kPrefix * sizeof(Cand16), hipMemcpyDeviceToHost, s),
@@ -496,7 +532,7 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
     if (k < np) {
       psx_ok(psx::fft::exec_fused_pass(plan, k, sl.d_desc, n, Z, s, rows_out), "fused pass");
     } else if (k == np) {
-      spectrum(sl, n);
+      spectrum(sl, n, !fused);
     } else {
       harmonics(sl, n);
     }

@@ -55,6 +55,8 @@ struct BoundAudit {
   uint64_t mismatches = 0;    // entries differing from the bound recomputed on the host from P
   uint64_t fundamentals = 0;  // bins above threshold in the fundamental's window (host scan of P)
   uint64_t appended = 0;      // candidates the device appended (the fundamental's, before the harmonic pass)
+  uint64_t crossing = 0;      // bins of the harmonic window whose bound crosses at some fold (summed exactly)
+  uint64_t window = 0;        // bins of the harmonic window (all trials)
 };
 
 struct StageTimes {
@@ -144,7 +146,8 @@ class Engine {
   void enqueue(Slot& sl, uint32_t n);
   // enqueue's body; run = false only builds the slot's graph for n
   void launch(Slot& sl, uint32_t n, bool run);
-  void spectrum(Slot& sl, uint32_t n);
+  // write_p: the spectra P too (always but in the fused bound form)
+  void spectrum(Slot& sl, uint32_t n, bool write_p);
   void harmonics(Slot& sl, uint32_t n);
   // the fused FFT leaves the transposed row layout (rows_R rows of rows_L)
   bool rows_out = false;
@@ -162,8 +165,12 @@ class Engine {
   psx::fft::RootTable rt{};
   float2* Z = nullptr;
   float* P = nullptr;
-  // bound-filtered harmonic pass: Q rows of qstride bfloat16 bounds
+  // bound-filtered harmonic pass: Q rows of qstride bfloat16 bounds;
+  // fused: the row-layout spectrum pass writes them (and no P, which the
+  // harmonic pass's exact sums recompute from Z)
   bool bound = true;
+  bool fused = false;
+  void* rows_src = nullptr;  // device copy of the rows source (fused form)
   int64_t qstride = 0;
   uint16_t* Q = nullptr;
   Windows win{};

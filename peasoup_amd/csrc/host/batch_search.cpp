@@ -271,6 +271,8 @@ extern "C" int psx_batch_bound_audit(const psx_search_args* args, const psx_batc
     result[1] = a.mismatches;
     result[2] = a.fundamentals;
     result[3] = a.appended;
+    result[4] = a.crossing;
+    result[5] = a.window;
     eng.reset();
     (void)hipFree(d_tim);
     (void)hipStreamDestroy(s);

@@ -30,15 +30,15 @@ def test_stage_bytes_model():
     # the fused trial moves 24n bytes with two FFT passes (about 28n with three)
     assert abs(sum(two) - 24 * n) <= 16
     # bound-filtered harmonic form: one 16-bit bound per two bins (n/2 + 2
-    # bytes) written with the spectrum (by a pooling pass that reads the
-    # spectrum again after the natural layout), read once by the harmonic pass
+    # bytes), read once by the harmonic pass; after the natural layout a
+    # pooling pass reads the spectrum again to write them
     qb = nb + 1
     bound = bench.stage_bytes(n, 2, fused=False)
     assert bound[:2] == two[:2]
     assert bound[2] == 8 * m + 8 * nb + qb and bound[3] == qb
-    # fused into the row-layout spectrum pass: no second read of the spectrum
+    # fused into the row-layout spectrum pass, which then writes no spectrum
     fused = bench.stage_bytes(n, 2)
-    assert fused[2] == 8 * m + 4 * nb + qb and fused[3] == qb
+    assert fused[2] == 8 * m + qb and fused[3] == qb
 
 
 def test_cpu_baseline_threads_and_model():

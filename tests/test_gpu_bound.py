@@ -57,6 +57,25 @@ def test_bound_equals_exact_low_thresholds(psx, thr):
     assert same(bound, ref)
 
 
+@pytest.mark.parametrize("thr", [1.5, 3.0, 4.5])
+def test_bound_rows_layout_low_thresholds(psx, thr):
+    """The row-layout form writes no spectrum: the bins whose bound crosses
+    are recomputed from the FFT output for the exact sums.  At these
+    thresholds most or many bins take that path, and every candidate's S/N
+    must equal the exact-gather form's (which reads the spectrum written by
+    the same pass) bit for bit."""
+    import peasoup_amd.workloads as W
+    n = 1 << 18
+    w = W.Workload("rlo", 2, n, 64e-6, 0x5EB0E0, 5.0, -20.0, 20.0, min_snr=thr,
+                   injections=[W.Injection(1, 0.0123, 5.0, 20.0, 0.3)])
+    series = W.generate_host(w)
+    accs = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)[:8]
+    bound, exact = both_forms(W, w, series, accs, 8, decomp=[(128, 16), (1024, 4)], spectrum_layout=W.SPECTRUM_ROWS)
+    print("rows thr %.1f: %d candidates" % (thr, len(exact)), flush=True)
+    assert len(exact) > 10
+    assert same(bound, exact) and len(bound) == len(exact)
+
+
 @pytest.mark.parametrize("layout", ["natural", "rows"])
 def test_bound_equals_exact_both_layouts(psx, layout):
     import peasoup_amd.workloads as W
@@ -115,13 +134,14 @@ def test_bound_spectrum_audit(psx, n, decomp, layout):
     psx.register_signatures({"psx_batch_bound_audit": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
                                                                  C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
                                                                  C.c_void_p])})
-    res = np.zeros(4, np.uint64)
+    res = np.zeros(6, np.uint64)
     opts = W.make_batch_opts(8, decomp=decomp, spectrum_layout=lay)
     psx.check(psx.lib().psx_batch_bound_audit(C.byref(w.search_args()), C.byref(opts), d.ptr, n,
                                               accs.ctypes.data, accs.size, res.ctypes.data))
-    entries, mismatches, fundamentals, appended = (int(x) for x in res)
-    print("n %d %s %s: %d entries, %d mismatches, %d fundamentals, %d appended"
-          % (n, decomp, layout, entries, mismatches, fundamentals, appended), flush=True)
+    entries, mismatches, fundamentals, appended, crossing, window = (int(x) for x in res)
+    print("n %d %s %s: %d entries, %d mismatches, %d fundamentals, %d appended, %d of %d bins cross"
+          % (n, decomp, layout, entries, mismatches, fundamentals, appended, crossing, window), flush=True)
+    assert window > 0 and crossing <= window
     assert entries == accs.size * ((n // 2 + 2) // 2)
     assert mismatches == 0
     assert fundamentals > 0 and appended == fundamentals
