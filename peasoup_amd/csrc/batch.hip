@@ -62,6 +62,28 @@ __device__ inline uint16_t bf16_up(float x) {
 }
 __device__ inline float bf16_value(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// Reads the kBoundD LDS entries e[0], e[STRIDE], ... into the high halves of
+// t[0..kBoundD), whose low halves the caller keeps at zero: ds_read_u16_d16_hi
+// writes only the high half, so t[d] is the entry's float value with no
+// conversion instruction.  The reads and their wait are one asm statement.
+template <int STRIDE>
+__device__ inline void lds_bf16x8(const uint16_t* e, float (&t)[8]) {
+  const uint32_t a = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint16_t*)e;
+  asm volatile(
+      "ds_read_u16_d16_hi %0, %8\n\t"
+      "ds_read_u16_d16_hi %1, %8 offset:%9\n\t"
+      "ds_read_u16_d16_hi %2, %8 offset:%10\n\t"
+      "ds_read_u16_d16_hi %3, %8 offset:%11\n\t"
+      "ds_read_u16_d16_hi %4, %8 offset:%12\n\t"
+      "ds_read_u16_d16_hi %5, %8 offset:%13\n\t"
+      "ds_read_u16_d16_hi %6, %8 offset:%14\n\t"
+      "ds_read_u16_d16_hi %7, %8 offset:%15\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "+v"(t[0]), "+v"(t[1]), "+v"(t[2]), "+v"(t[3]), "+v"(t[4]), "+v"(t[5]), "+v"(t[6]), "+v"(t[7])
+      : "v"(a), "i"(2 * STRIDE), "i"(4 * STRIDE), "i"(6 * STRIDE), "i"(8 * STRIDE), "i"(10 * STRIDE),
+        "i"(12 * STRIDE), "i"(14 * STRIDE));
+}
+
 // Outputs of the spectrum pass for the bound-filtered harmonic pass: the
 // bound spectrum (one bfloat16 upper bound per bin pair 2u, 2u+1) and
 // the fundamental's above-threshold bins (device_find_peaks over
@@ -694,20 +716,22 @@ struct BoundRuns {
   // adds harmonic H's bound to every running sum; at a fold's last addend
   // marks the bins whose bound crosses that fold's threshold
   __device__ static void add(const uint16_t* my, const uint32_t (&a0)[NH + 1], uint32_t i0, const FoldThr& ft,
-                             float (&val)[kBoundD], bool (&fail)[kBoundD]) {
+                             float (&val)[kBoundD], bool (&fail)[kBoundD], float (&t)[kBoundD]) {
     if constexpr (H < NH) {
       constexpr int K = hK(H), J = hJ(H);
       constexpr uint32_t c = 1u << (K - 1);
       constexpr uint32_t step = (64u * J) >> K;  // source step between a lane's bins
       static_assert(step % kBoundPool == 0, "constant LDS offsets per d");
       const uint16_t* b = my + run_offset(H) + (((J * i0 + c) >> K) / kBoundPool - a0[H]);
+      static_assert(kBoundD == 8, "lds_bf16x8 reads eight entries");
+      lds_bf16x8<step / kBoundPool>(b, t);
 #pragma unroll
-      for (int d = 0; d < kBoundD; ++d) val[d] += bf16_value(b[d * (step / kBoundPool)]);
+      for (int d = 0; d < kBoundD; ++d) val[d] += t[d];
       if constexpr (((H + 2) & (H + 1)) == 0) {  // H = 2^K - 2: fold K complete
 #pragma unroll
         for (int d = 0; d < kBoundD; ++d) fail[d] |= val[d] > ft.t[K];
       }
-      BoundRuns<NH, H + 1>::add(my, a0, i0, ft, val, fail);
+      BoundRuns<NH, H + 1>::add(my, a0, i0, ft, val, fail, t);
     }
   }
 };
@@ -776,6 +800,11 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(bound_w
   uint16_t* my = lds + wave * WE;
   const uint32_t hi = (uint32_t)w.hi;
   const uint32_t wstep = gridDim.x * (kB / 64) * kBoundWaveBins;
+  // registers the bound entries are read into, high halves only: their low
+  // halves stay zero for the whole kernel
+  float t[kBoundD];
+#pragma unroll
+  for (int d = 0; d < kBoundD; ++d) t[d] = 0.f;
   for (uint32_t base = (uint32_t)w.lo + (bx * (kB / 64) + wave) * kBoundWaveBins; base < hi; base += wstep) {
     const uint32_t i0 = base + lane;
     // the previous tile's LDS reads are complete before the area is refilled
@@ -786,15 +815,13 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(bound_w
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     float val[kBoundD];
     bool fail[kBoundD];
-    {
-      const uint16_t* b = my + (i0 / kBoundPool - a0[NH]);
+    lds_bf16x8<64 / kBoundPool>(my + (i0 / kBoundPool - a0[NH]), t);
 #pragma unroll
-      for (int d = 0; d < kBoundD; ++d) {
-        val[d] = bf16_value(b[d * (64 / kBoundPool)]);
-        fail[d] = false;
-      }
+    for (int d = 0; d < kBoundD; ++d) {
+      val[d] = t[d];
+      fail[d] = false;
     }
-    BoundRuns<NH, 0>::add(my, a0, i0, ft, val, fail);
+    BoundRuns<NH, 0>::add(my, a0, i0, ft, val, fail, t);
     bool any = false;
 #pragma unroll
     for (int d = 0; d < kBoundD; ++d) any |= fail[d];
