@@ -388,13 +388,14 @@ def main():
         P.register_signatures({"psx_batch_bound_audit": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
                                                                     C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
                                                                     C.c_void_p])})
-        res = np.zeros(6, np.uint64)
+        res = np.zeros(7, np.uint64)
         acc8 = np.ascontiguousarray(accs[:8], np.float32)
         P.check(P.lib().psx_batch_bound_audit(C.byref(args_c), C.byref(opts), d_series.ptr, w.size,
                                               acc8.ctypes.data, acc8.size, res.ctypes.data))
         bound_stats = {"bins": int(res[5]), "bins_summed_exactly": int(res[4]),
                        "fraction_summed_exactly": float(res[4]) / max(1, int(res[5])),
-                       "bound_entries_checked": int(res[0]), "bound_entries_wrong": int(res[1]),
+                       "bound_entries_checked": int(res[0]), "bound_entries_invalid": int(res[1]),
+                       "bound_entries_loose": int(res[6]),
                        "sample": "DM 0, %d accelerations" % acc8.size}
         log("bound audit", bound_stats)
 

@@ -315,15 +315,18 @@ int psx_batch_profile(const psx_search_args* args, const psx_batch_opts* opts, c
 
 /* Self-check of the bound-filtered harmonic pass's input (no counterpart
  * in the reference): runs the prologue of d_series and the FFT and spectrum
- * stages of one batch (min(naccs, batch) trials), then recomputes on the host,
- * from the spectra P, every bound entry (the bfloat16 value rounded up
- * from max(P[2u], P[2u+1])) and the fundamental's above-threshold bins.
- * result[0] = entries checked, [1] = entries that differ (0 when correct),
- * [2] = fundamental bins above threshold on the host, [3] = candidates the
- * device appended (equal to [2] when correct), [4] = bins of the harmonic
- * window whose bound sums cross at some fold (the bins summed exactly),
- * [5] = bins of the window (result holds 6 values).  For tests and for
- * validating a new FFT decomposition. */
+ * stages of one batch (min(naccs, batch) trials), has the exact spectra P
+ * written from the same FFT output, and checks on the host every bound
+ * entry against max(P[2u], P[2u+1]) and the fundamental's candidates against
+ * the bins above threshold.  result[0] = entries checked, [1] = entries below
+ * their pair's maximum (invalid bounds: 0 when correct), [2] = fundamental
+ * bins above threshold on the host, [3] = candidates the device appended
+ * (equal to [2] when correct), [4] = bins of the harmonic window whose bound
+ * sums cross at some fold (the bins summed exactly), [5] = bins of the
+ * window, [6] = valid entries above the tight bound (the pair maximum
+ * rounded up to bfloat16; the row-layout pass bounds each value from
+ * approximate arithmetic and may land one step higher).  result holds 7
+ * values.  For tests and for validating a new FFT decomposition. */
 int psx_batch_bound_audit(const psx_search_args* args, const psx_batch_opts* opts, const float* d_series,
                           uint64_t nsamps, const float* accs, uint32_t naccs, uint64_t* result);
 

@@ -46,6 +46,26 @@ __device__ inline float normalised(float2 x, float2 prev, float mean_n, float st
   return v;
 }
 
+// An upper bound of normalised(x, prev, mean_n, std_n) (std_n > 0) from the
+// hardware's approximate square root and reciprocal (each within 1 ulp):
+// the interbin square root is raised by 2^-20 of itself (8 ulp, more than
+// the approximation's error), the mean subtracted as the reference does
+// (rounding is monotone), and the quotient taken as a product with the
+// approximate reciprocal and raised by 2^-20 of its magnitude (its error is
+// under 2 ulp of the quotient, plus the half ulp of correct rounding).
+// About a third of normalised's instructions: the bound-form spectrum pass
+// needs the exact value only where this one exceeds the threshold.
+__device__ inline float normalised_up(float2 x, float2 prev, float mean_n, float std_n) {
+  const float ampsq = fmaf(x.x, x.x, x.y * x.y);
+  const float dr = x.x - prev.x;
+  const float di = x.y - prev.y;
+  const float ampsq_diff = 0.5f * fmaf(dr, dr, di * di);
+  const float a = __builtin_amdgcn_sqrtf(fmaxf(ampsq, ampsq_diff));
+  const float v = fmaf(a, 0x1p-20f, a) - mean_n;
+  const float q = v * __builtin_amdgcn_rcpf(std_n);
+  return fmaf(fabsf(q), 0x1p-20f, q);
+}
+
 __device__ inline float2 shfl_f2(float2 v, int src) {
   return make_float2(__shfl(v.x, src), __shfl(v.y, src));
 }
@@ -221,7 +241,7 @@ __device__ inline float2 x_rows(const float2* y, uint32_t R, uint32_t L, uint32_
 // value; with bo, also the bound of the bin pair (k, k+1), whose second bin
 // (row 1 or R/2 + 1, a tile's) is recomputed here through the same untangle
 // and interbin arithmetic, so it equals the tile's value bit for bit.
-template <bool BOUND, bool WRITE_P>
+template <bool BOUND>
 __device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uint32_t L, uint32_t t,
                                       const RootTable& rt, float mean_n, float std_n, uint16_t* q, uint32_t& k) {
   const uint32_t M = R * L;
@@ -236,7 +256,7 @@ __device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uin
   }
   if (k > 0) prev = x_rows(y, R, L, k - 1, rt);
   const float v = normalised(x, prev, mean_n, std_n);
-  if constexpr (WRITE_P) p[k] = v;
+  if constexpr (!BOUND) p[k] = v;
   if constexpr (BOUND) {
     float m = v;  // bin M has no partner (n/2 + 1 bins)
     if (k < M) m = fmaxf(v, normalised(x_rows(y, R, L, k + 1, rt), x, mean_n, std_n));
@@ -245,10 +265,11 @@ __device__ inline float spec_edge_bin(const float2* y, float* p, uint32_t R, uin
   return v;
 }
 
-// BOUND: also the bound spectrum and the fundamental's candidates; WRITE_P:
-// the spectra themselves (the bound-filtered harmonic pass recomputes the
-// few bins it sums exactly from Y instead).
-template <bool HI_FIXED, bool BOUND, bool WRITE_P>
+// BOUND: the bound spectrum and the fundamental's candidates instead of the
+// spectra (the bound-filtered harmonic pass recomputes the few bins it sums
+// exactly from Y); the tile's values are then upper bounds (normalised_up),
+// and a value above the threshold is recomputed exactly for the fundamental.
+template <bool HI_FIXED, bool BOUND>
 __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, float* __restrict__ P, uint32_t R,
                                                   uint32_t L, int64_t pstride, RootTable rt,
                                                   const TrialDesc* __restrict__ descs, BoundOut bo) {
@@ -262,13 +283,19 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   float* p = P + (int64_t)tr * pstride;
   const float mean_n = descs[tr].mean_n, std_n = descs[tr].std_n;
   uint16_t* q = BOUND ? bo.Q + (int64_t)tr * bo.qstride : nullptr;
+  auto tile_value = [&](float2 xv, float2 pv) {
+    if constexpr (BOUND)
+      return normalised_up(xv, pv, mean_n, std_n);
+    else
+      return normalised(xv, pv, mean_n, std_n);
+  };
   if (blockIdx.y == gridDim.y - 1) {
     // the edge bins: L/64 workgroups of kB threads cover the 2L+1 of them
     const uint32_t t = blockIdx.x * kB + threadIdx.x;
     const bool valid = t <= 2 * L;
     uint32_t k = 0;
     float v = 0.f;
-    if (valid) v = spec_edge_bin<BOUND, WRITE_P>(y, p, R, L, t, rt, mean_n, std_n, q, k);
+    if (valid) v = spec_edge_bin<BOUND>(y, p, R, L, t, rt, mean_n, std_n, q, k);
     if constexpr (BOUND) append_fundamental(valid && fundamental_hit(v, k, bo), descs[tr].slot, k, v, bo);
     return;
   }
@@ -341,8 +368,8 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
       xlo_first = xlo;
       edge_hi[wv][lane] = xhi;
     } else {
-      lo_t[lane][slot(qq)] = normalised(xlo, xlo_prev, mean_n, std_n);       // P(q)
-      up_t[lane][slot(qq - 1)] = normalised(xhi_prev, xhi, mean_n, std_n);   // P(R-(q-1))
+      lo_t[lane][slot(qq)] = tile_value(xlo, xlo_prev);       // P(q)
+      up_t[lane][slot(qq - 1)] = tile_value(xhi_prev, xhi);   // P(R-(q-1))
     }
     xlo_prev = xlo;
     xhi_prev = xhi;
@@ -355,15 +382,15 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
   __syncthreads();
   if (wv > 0) below = edge_lo[wv - 1][lane];
   if (wv < kB / 64 - 1) above = edge_hi[wv + 1][lane];
-  lo_t[lane][slot(r0)] = normalised(xlo_first, below, mean_n, std_n);
-  up_t[lane][slot(r0 + kWaveRows - 1)] = normalised(xhi_last, above, mean_n, std_n);
+  lo_t[lane][slot(r0)] = tile_value(xlo_first, below);
+  up_t[lane][slot(r0 + kWaveRows - 1)] = tile_value(xhi_last, above);
   if constexpr (BOUND) {
     // P of the next tile's first row (ra + 64), the partner of this tile's
     // last row in a bound pair: its X was loaded for the upper halo, and the
     // interbin neighbour is this tile's last row.  Slot kRowTile is free in
     // every tile but the last, which has no such pair.
     if (wv == kB / 64 - 1 && ra + kRowTile < half_rows)
-      lo_t[lane][kRowTile] = normalised(next_lo, xlo_prev, mean_n, std_n);
+      lo_t[lane][kRowTile] = tile_value(next_lo, xlo_prev);
   }
   __syncthreads();
   const uint32_t rr = threadIdx.x & 63u;
@@ -394,32 +421,35 @@ __global__ void __launch_bounds__(kB) k_spec_rows(const float2* __restrict__ Y, 
     bool any = false;
 #pragma unroll 4
     for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
-      const uint32_t kl = r + R * (ea + j);
-      const uint32_t ku = (R - r) + R * (L - 1 - (ea + j));
       const float vl = lo_t[j][rr], vu = up_t[j][rr];
-      if (rv) {
-        if constexpr (WRITE_P) {
-          stg1(p, kl, vl);
-          stg1(p, ku, vu);
-        }
-        any |= fmaxf(vl, vu) > bo.thr;
-      }
+      if (rv) any |= fmaxf(vl, vu) > bo.thr;
       if (pv) {
         const float m = lower ? fmaxf(lo_t[j][s0], lo_t[j][s1]) : fmaxf(up_t[j][s0], up_t[j][s1]);
         const uint32_t k2 = lower ? rpair + R * (ea + j) : (R - rpair) + R * (L - 1 - (ea + j));
         q[k2 / 2] = bf16_up(m);
       }
     }
-    // above-threshold values are rare: a second sweep over the tile appends
-    // them, in the waves that have any
+    // bounds above the threshold are rare: a second sweep over the tile, in
+    // the waves that have any, recomputes those bins exactly (the same
+    // untangle, interbin and normalisation of the same inputs as the exact
+    // pass) and appends the ones above threshold
     if (__ballot(any) != 0) {
       const uint32_t slot_id = descs[tr].slot;
+      const uint32_t rshift = HI_FIXED ? (uint32_t)__builtin_ctz(R) : 32u;
+      auto exact = [&](uint32_t k) {
+        const float2 prev = k > 0 ? x_rows(y, R, L, k - 1, rt, rshift) : make_float2(0.f, 0.f);
+        return normalised(x_rows(y, R, L, k, rt, rshift), prev, mean_n, std_n);
+      };
       for (uint32_t j = wv; j < (uint32_t)kColTile; j += kB / 64) {
         const uint32_t kl = r + R * (ea + j);
         const uint32_t ku = (R - r) + R * (L - 1 - (ea + j));
-        const float vl = lo_t[j][rr], vu = up_t[j][rr];
-        append_fundamental(rv && fundamental_hit(vl, kl, bo), slot_id, kl, vl, bo);
-        append_fundamental(rv && fundamental_hit(vu, ku, bo), slot_id, ku, vu, bo);
+        float vl = lo_t[j][rr], vu = up_t[j][rr];
+        const bool cl = rv && fundamental_hit(vl, kl, bo);
+        const bool cu = rv && fundamental_hit(vu, ku, bo);
+        if (cl) vl = exact(kl);
+        if (cu) vu = exact(ku);
+        append_fundamental(cl && vl > bo.thr, slot_id, kl, vl, bo);
+        append_fundamental(cu && vu > bo.thr, slot_id, ku, vu, bo);
       }
     }
   }
@@ -935,14 +965,11 @@ int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t 
                   bound->w.thr, bound->out, bound->cap, bound->count};
   const bool hi_fixed = (R & (R - 1)) == 0 && R <= (1u << rt.shift);
   if (!bound && !P) return psx::fail(PSX_ERR_ARG, "spec_rows: no spectrum buffer");
-  const bool write_p = P != nullptr;
-#define PSX_SPEC_ROWS(HF, BD, WP) k_spec_rows<HF, BD, WP><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo)
+#define PSX_SPEC_ROWS(HF, BD) k_spec_rows<HF, BD><<<grid, kB, 0, s>>>(Y, P, R, L, pstride, rt, d_descs, bo)
   if (!bound)
-    hi_fixed ? PSX_SPEC_ROWS(true, false, true) : PSX_SPEC_ROWS(false, false, true);
-  else if (write_p)
-    hi_fixed ? PSX_SPEC_ROWS(true, true, true) : PSX_SPEC_ROWS(false, true, true);
+    hi_fixed ? PSX_SPEC_ROWS(true, false) : PSX_SPEC_ROWS(false, false);
   else
-    hi_fixed ? PSX_SPEC_ROWS(true, true, false) : PSX_SPEC_ROWS(false, true, false);
+    hi_fixed ? PSX_SPEC_ROWS(true, true) : PSX_SPEC_ROWS(false, true);
 #undef PSX_SPEC_ROWS
   PSX_CHECK_LAUNCH("k_spec_rows");
   return PSX_OK;

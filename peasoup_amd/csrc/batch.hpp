@@ -42,9 +42,9 @@ struct SpecBound {
 };
 // Spectrum pass over the transposed layout of a row-writing last FFT pass
 // (row r of L elements holds Z[r + R*e]); same output as launch_spec_post.
-// bound (optional): also writes Q and the fundamental's candidates, which
-// launch_pool_peaks would otherwise compute from P; with bound, P may be
-// nullptr (no spectra written).
+// bound (optional): writes Q and the fundamental's candidates instead of P
+// (which may then be nullptr), as launch_pool_peaks would from P; its bound
+// entries may be one bfloat16 step above those of the exact spectra.
 int launch_spec_rows(const float2* Y, float* P, uint32_t R, uint32_t L, int64_t pstride,
                      const psx::fft::RootTable& rt, const psx::fft::TrialDesc* d_descs, uint32_t ntr, hipStream_t s,
                      const SpecBound* bound = nullptr);

@@ -180,10 +180,10 @@ void Engine::prologue(const float* d_series, uint64_t n_real, float* d_tim, floa
 // In the bound-filtered form the row-layout pass also writes the bound
 // spectrum and the fundamental's candidates; after the natural-layout pass a
 // pooling pass computes them from P.
-void Engine::spectrum(Slot& sl, uint32_t n, bool write_p) {
+void Engine::spectrum(Slot& sl, uint32_t n) {
   if (rows_out) {
     const SpecBound sb{Q, qstride, win, sl.d_cands, cap, sl.d_count};
-    psx_ok(launch_spec_rows(Z, write_p ? P : nullptr, rows_R, rows_L, pstride, rt, sl.d_desc, n, s,
+    psx_ok(launch_spec_rows(Z, fused ? nullptr : P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s,
                             fused ? &sb : nullptr),
            "spec_rows");
   } else {
@@ -218,13 +218,14 @@ void Engine::audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out) 
   if (!pending.empty() || !ready.empty()) throw std::runtime_error("audit_bounds: engine busy");
   Slot& sl = slots[0];
   fill_slot(sl, trials, n);
-  // the fused form writes no spectra: the audit has them written beside the
-  // bounds, by the same pass
+  // the fused form writes no spectra: the audit has the exact-form spectrum
+  // pass write them from the same FFT output after the bound-form pass
   if (!P) dalloc(&P, (uint64_t)pstride * B + kHarmWaveBins);
   hip_ok(hipMemcpyAsync(sl.d_desc, sl.h_desc, n * sizeof(psx::fft::TrialDesc), hipMemcpyHostToDevice, s), "desc");
   psx_ok(launch_zero(sl.d_count, s), "zero");
   psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
-  spectrum(sl, n, true);
+  spectrum(sl, n);
+  if (fused) psx_ok(launch_spec_rows(Z, P, rows_R, rows_L, pstride, rt, sl.d_desc, n, s, nullptr), "spec_rows");
   std::vector<float> hp((size_t)pstride * n);
   std::vector<uint16_t> hq((size_t)qstride * n);
   uint32_t count = 0;
@@ -241,7 +242,15 @@ void Engine::audit_bounds(const TrialWork* trials, uint32_t n, BoundAudit& out) 
       const float a = pr[2 * u];
       const float m = 2 * u + 1 < nb ? std::max(a, pr[2 * u + 1]) : a;
       out.entries++;
-      if (qr[u] != bf16_up_bits(m)) out.mismatches++;
+      // a valid bound: its float value is at least the pair's maximum; the
+      // tight one is bf16_up_bits(m), and the fused pass (which bounds each
+      // value from approximate arithmetic) may be one step above it
+      const uint16_t tight = bf16_up_bits(m);
+      const uint32_t bits = (uint32_t)qr[u] << 16;
+      float qv;
+      std::memcpy(&qv, &bits, 4);
+      if (!(qv >= m)) out.mismatches++;
+      else if (qr[u] != tight) out.loose++;
     }
     for (uint64_t k = (uint64_t)std::max(win.start[0], 0); k < (uint64_t)std::max(win.end[0], 0); ++k)
       if (pr[k] > win.thr) out.fundamentals++;
@@ -332,7 +341,7 @@ void Engine::launch(Slot& sl, uint32_t n, bool run) {
            "descriptor upload");
     psx_ok(launch_zero(sl.d_count, s), "zero");
     psx_ok(psx::fft::exec_fused_r2c(plan, sl.d_desc, n, Z, s, rows_out), "fused FFT");
-    spectrum(sl, n, !fused);
+    spectrum(sl, n);
     harmonics(sl, n);
     hip_ok(hipMemcpyAsync(sl.h_count, sl.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "count copy");
     hip_ok(hipMemcpyAsync(sl.h_cands, sl.d_cands, kPrefix * sizeof(Cand16), hipMemcpyDeviceToHost, s),
@@ -532,7 +541,7 @@ void Engine::profile(const TrialWork* trials, uint32_t n, int reps, StageTimes& 
     if (k < np) {
       psx_ok(psx::fft::exec_fused_pass(plan, k, sl.d_desc, n, Z, s, rows_out), "fused pass");
     } else if (k == np) {
-      spectrum(sl, n, !fused);
+      spectrum(sl, n);
     } else {
       harmonics(sl, n);
     }

@@ -52,7 +52,8 @@ struct EngineOptions {
 // Self-check of the bound spectrum one batch leaves (Engine::audit_bounds).
 struct BoundAudit {
   uint64_t entries = 0;       // bound entries checked (all trials)
-  uint64_t mismatches = 0;    // entries differing from the bound recomputed on the host from P
+  uint64_t mismatches = 0;    // entries below the pair maximum of P (invalid bounds)
+  uint64_t loose = 0;         // valid entries above the tight bound (bf16 rounding up of the maximum)
   uint64_t fundamentals = 0;  // bins above threshold in the fundamental's window (host scan of P)
   uint64_t appended = 0;      // candidates the device appended (the fundamental's, before the harmonic pass)
   uint64_t crossing = 0;      // bins of the harmonic window whose bound crosses at some fold (summed exactly)
@@ -146,8 +147,7 @@ class Engine {
   void enqueue(Slot& sl, uint32_t n);
   // enqueue's body; run = false only builds the slot's graph for n
   void launch(Slot& sl, uint32_t n, bool run);
-  // write_p: the spectra P too (always but in the fused bound form)
-  void spectrum(Slot& sl, uint32_t n, bool write_p);
+  void spectrum(Slot& sl, uint32_t n);
   void harmonics(Slot& sl, uint32_t n);
   // the fused FFT leaves the transposed row layout (rows_R rows of rows_L)
   bool rows_out = false;

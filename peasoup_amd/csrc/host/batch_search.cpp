@@ -273,6 +273,7 @@ extern "C" int psx_batch_bound_audit(const psx_search_args* args, const psx_batc
     result[3] = a.appended;
     result[4] = a.crossing;
     result[5] = a.window;
+    result[6] = a.loose;
     eng.reset();
     (void)hipFree(d_tim);
     (void)hipStreamDestroy(s);

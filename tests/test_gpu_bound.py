@@ -119,11 +119,12 @@ def test_bound_headline_length_matches_worker(psx):
     (1 << 23, [(2048, 8), (2048, 4)], "rows"),
 ])
 def test_bound_spectrum_audit(psx, n, decomp, layout):
-    """Every bound entry the spectrum stage leaves equals the bfloat16
-    value rounded up from max(P[2u], P[2u+1]) recomputed on the host from the
-    same spectra (row-layout tiles, their edge rows and bin n/2, and the
-    pooling pass after the natural layout), and the fundamental's appended
-    candidates are exactly the bins above threshold in its window."""
+    """Every bound entry the spectrum stage leaves is at least max(P[2u],
+    P[2u+1]) of the exact spectra of the same FFT output (row-layout tiles,
+    their edge rows and bin n/2, and the pooling pass after the natural
+    layout), rarely one bfloat16 step above the tight value, and the
+    fundamental's appended candidates are exactly the bins above threshold
+    in its window."""
     import ctypes as C
     import peasoup_amd.workloads as W
     lay = {"auto": W.SPECTRUM_AUTO, "rows": W.SPECTRUM_ROWS, "natural": W.SPECTRUM_NATURAL}[layout]
@@ -134,14 +135,16 @@ def test_bound_spectrum_audit(psx, n, decomp, layout):
     psx.register_signatures({"psx_batch_bound_audit": (C.c_int, [C.POINTER(W.SearchArgs), C.POINTER(W.BatchOpts),
                                                                  C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
                                                                  C.c_void_p])})
-    res = np.zeros(6, np.uint64)
+    res = np.zeros(7, np.uint64)
     opts = W.make_batch_opts(8, decomp=decomp, spectrum_layout=lay)
     psx.check(psx.lib().psx_batch_bound_audit(C.byref(w.search_args()), C.byref(opts), d.ptr, n,
                                               accs.ctypes.data, accs.size, res.ctypes.data))
-    entries, mismatches, fundamentals, appended, crossing, window = (int(x) for x in res)
-    print("n %d %s %s: %d entries, %d mismatches, %d fundamentals, %d appended, %d of %d bins cross"
-          % (n, decomp, layout, entries, mismatches, fundamentals, appended, crossing, window), flush=True)
+    entries, invalid, fundamentals, appended, crossing, window, loose = (int(x) for x in res)
+    print("n %d %s %s: %d entries, %d invalid, %d loose, %d fundamentals, %d appended, %d of %d bins cross"
+          % (n, decomp, layout, entries, invalid, loose, fundamentals, appended, crossing, window), flush=True)
     assert window > 0 and crossing <= window
+    # the bounds of approximate arithmetic rarely land a step above the tight one
+    assert loose <= entries // 100
     assert entries == accs.size * ((n // 2 + 2) // 2)
-    assert mismatches == 0
+    assert invalid == 0
     assert fundamentals > 0 and appended == fundamentals
