@@ -281,6 +281,8 @@ def main():
     ap.add_argument("--ndm", type=int, default=64)
     ap.add_argument("--batch", type=int, default=0, help="trials per fused batch (0 = tuned/default)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--decomp", default="",
+                    help="FFT passes as len/cols,len/cols[,...] (default: the tuned or library default plan)")
     ap.add_argument("--harmonic", choices=("bound", "exact"), default="bound",
                     help="harmonic pass: bound-filtered (default) or exact gathers of every addend")
     ap.add_argument("--no-keep", action="store_true",
@@ -319,10 +321,13 @@ def main():
     tuned = tuning.lookup(w.size, dev["name"])
     batch = a.batch or tuned.get("batch", 16)
     bound = a.harmonic == "bound"
-    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=tuned.get("decomp"), keep_engine=not a.no_keep,
+    decomp = tuned.get("decomp")
+    if a.decomp:
+        decomp = [tuple(int(x) for x in p.split("/")) for p in a.decomp.split(",")]
+    opts = W.make_batch_opts(batch, use_graph=not a.no_graph, decomp=decomp, keep_engine=not a.no_keep,
                              harmonic_pass=W.HARMONIC_BOUND if bound else W.HARMONIC_EXACT)
     log("workload %s: %d DMs x %d accels = %d trials, batch %d, decomp %s" %
-        (w.name, w.ndm, accs.size, ntrials, batch, tuned.get("decomp")))
+        (w.name, w.ndm, accs.size, ntrials, batch, decomp))
 
     if a.profile_run:
         w.ndm = 1
