@@ -302,10 +302,11 @@ struct PassArgs {
   int32_t resample;
   uint64_t rs_n;
   const TrialDesc* descs;
-  // Two column tiles per workgroup (kernels where two_tiles_ok): the grid
-  // covers ceil(ntiles/2) tiles and workgroup b also does tile b + gridDim.x
-  // (< ntiles), its loads issued before the first tile is transformed.
-  int32_t tiles2;
+  // Several column tiles per workgroup (kernels where two_tiles_ok): the
+  // grid covers ceil(ntiles/tiles) tiles and workgroup b also does tiles
+  // b + gridDim.x, b + 2*gridDim.x, ... (< ntiles), each tile's loads issued
+  // before the previous one is transformed.  0 or 1: one tile.
+  int32_t tiles;
   uint32_t ntiles;
 };
 // Passes that may take two tiles per workgroup: contiguous tiles of at
