@@ -405,6 +405,7 @@ const KernelInfo kKernels[] = {
     PSX_KC(48, 16), PSX_KC(96, 16), PSX_KC(192, 16), PSX_KC(384, 16), PSX_KC(768, 16),
     PSX_KC(96, 8), PSX_KC(192, 8), PSX_KC(384, 8), PSX_KC(768, 8), PSX_KC(1536, 8),
     PSX_KC(384, 4), PSX_KC(768, 4), PSX_KC(1536, 4), PSX_KC(3072, 4),
+    PSX_KR(128, 16), PSX_KR(256, 16), PSX_KR(512, 16),
     PSX_KR(512, 8), PSX_KR(1024, 8), PSX_KR(2048, 8), PSX_KR(768, 8), PSX_KR(1536, 8),
     PSX_KR(1024, 4), PSX_KR(2048, 4), PSX_KR(4096, 4), PSX_KR(1536, 4), PSX_KR(3072, 4),
     PSX_KR(2048, 2), PSX_KR(4096, 2),
@@ -693,7 +694,7 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
     B.gx = (unsigned)(M2 / d.C[1]); B.gy = 1; B.gz = p->batch;
     p->passes.push_back(A);
     p->passes.push_back(B);
-    if (!c2r && d.C[1] < 16) {
+    if (!c2r) {
       PassLaunch Bt = make(1, 1, 2, PM_CONTIG);
       Bt.a.in_col = M1; Bt.a.in_elem = 1; Bt.a.in_sub = 0;
       Bt.a.out_col = M1; Bt.a.out_elem = 1; Bt.a.out_sub = 0;
@@ -724,6 +725,19 @@ int build_passes(psx_fft_plan* p, bool device_tables) {
     p->passes.push_back(A);
     p->passes.push_back(B);
     p->passes.push_back(Cc);
+    if (!c2r) {
+      // the same last pass writing contiguous rows: transform (c, y) to row
+      // c + Q*y of M1 elements, i.e. row r holds Z[r + (Q*M2)*e]
+      PassLaunch Ct = make(2, 1, 2, PM_CONTIG);
+      Ct.a.in_col = P; Ct.a.in_elem = 1; Ct.a.in_sub = M1;
+      Ct.a.out_col = M1; Ct.a.out_elem = 1; Ct.a.out_sub = Q * M1;
+      Ct.a.in_cf = 0; Ct.a.out_cf = 0;
+      Ct.gx = Cc.gx; Ct.gy = Cc.gy; Ct.gz = p->batch;
+      if (Ct.k) {
+        p->rows_last = Ct;
+        p->has_rows_last = true;
+      }
+    }
   }
   if (c2r) p->passes.back().a.out_conj = 1;
   for (const PassLaunch& L : p->passes) {
@@ -885,8 +899,11 @@ int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, 
 
 bool rows_layout(const psx_fft_plan* p, uint32_t* rows, uint32_t* row_len) {
   if (!p->has_rows_last) return false;
-  *rows = (uint32_t)p->d.L[0];
-  *row_len = (uint32_t)p->d.L[1];
+  const int last = p->d.npass - 1;
+  uint32_t r = 1;
+  for (int q = 0; q < last; ++q) r *= (uint32_t)p->d.L[q];
+  *rows = r;
+  *row_len = (uint32_t)p->d.L[last];
   return true;
 }
 

@@ -66,9 +66,10 @@ int exec_fused_r2c(psx_fft_plan* p, const TrialDesc* d_descs, uint32_t ntrials, 
                    bool rows_out = false);
 int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t ntrials, float2* zout, hipStream_t s,
                     bool rows_out = false);
-// True when a two-pass plan's last pass can write contiguous rows: row r of
-// row_len elements then holds Z[r + rows*e], e = 0..row_len-1 (rows = the
-// first pass's length), instead of the natural Z[k].
+// True when a plan's last pass can write contiguous rows: row r of row_len
+// elements (the last pass's length) then holds Z[r + rows*e],
+// e = 0..row_len-1 (rows = the product of the other passes' lengths),
+// instead of the natural Z[k].
 bool rows_layout(const psx_fft_plan* p, uint32_t* rows, uint32_t* row_len);
 int num_passes(const psx_fft_plan* p);
 // True when pass q transforms the work buffer in place (the middle pass of

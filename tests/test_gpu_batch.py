@@ -212,6 +212,9 @@ def _plan_desc(psx, W, w, series0, accs, opts):
     (3 << 19, [(384, 16), (2048, 8)]),        # R = 384: R/2 - 1 = 191, last tile partial
     (3 << 19, [(512, 16), (1536, 8)]),        # radix-3 row length
     (1 << 23, [(1024, 16), (4096, 4)]),       # the 2^23 workload's plan
+    (1 << 20, [(1024, 8), (512, 16)]),        # 16-column last pass (the tuned 2^20 plan)
+    (1 << 21, [(32, 16), (128, 16), (256, 16)]),   # three passes: R = 4096 rows of 256
+    (3 << 19, [(48, 16), (128, 16), (128, 16)]),   # three passes, radix-3 first: R = 6144
 ])
 def test_row_layout_equals_natural_layout(psx, n, decomp):
     import peasoup_amd.workloads as W
@@ -251,7 +254,8 @@ def test_row_layout_unavailable_is_an_error(psx):
     import peasoup_amd.workloads as W
     w = W.Workload("rows", 1, 1 << 17, 64e-6, 0x5EA5C8, 0.0, -5.0, 5.0)
     series = W.generate_host(w)
-    # a three-pass plan has no row-layout last pass
+    # no contiguous-row kernel exists for this plan's last pass (64 points,
+    # 16 columns), so it has no row-layout form
     opts = W.make_batch_opts(2, decomp=[(16 * 2, 16), (32, 16), (64, 16)], spectrum_layout=W.SPECTRUM_ROWS)
     with pytest.raises(psx.PsxError, match="row-layout"):
         W.batch_search(w.search_args(), series, w.dm_list(), None, opts)

@@ -110,13 +110,16 @@ def test_bound_headline_length_matches_worker(psx):
 
 @pytest.mark.parametrize("n,decomp,layout", [
     (1 << 17, None, "auto"),
-    (1 << 20, [(1024, 8), (512, 16)], "auto"),
+    (1 << 20, [(1024, 8), (512, 16)], "natural"),
     (1 << 21, [(1024, 8), (1024, 4)], "rows"),
     (1 << 22, [(1024, 8), (2048, 4)], "rows"),
     (3 << 21, [(2048, 8), (1536, 4)], "rows"),
     (3 << 21, [(2048, 8), (1536, 4)], "natural"),
     (1 << 23, [(1024, 16), (4096, 4)], "rows"),
     (1 << 23, [(2048, 8), (2048, 4)], "rows"),
+    (1 << 20, [(1024, 8), (512, 16)], "rows"),
+    (1 << 21, [(32, 16), (128, 16), (256, 16)], "rows"),
+    (3 << 19, [(48, 16), (128, 16), (128, 16)], "rows"),
 ])
 def test_bound_spectrum_audit(psx, n, decomp, layout):
     """Every bound entry the spectrum stage leaves is at least max(P[2u],
