@@ -465,7 +465,7 @@ def main():
     if not a.no_uneven or not a.no_serve or not a.no_long:
         del d_series
     if not a.no_long:
-        result["long_segment"] = long_segment(P, W)
+        result["long_segment"] = long_segment(P, W, batch=tuning.lookup(1 << 26, dev["name"]).get("batch", 16))
     if not a.no_uneven:
         result["uneven_batched"] = uneven_batched(P, W, batch=batch, schedules=tuning.table(dev["name"]))
     if not a.no_serve:

@@ -225,9 +225,23 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
   constexpr int NB0 = P / R0;
   constexpr bool cf = ModeTraits<MODE>::in_cf;
   extern __shared__ float lds[];
-  const unsigned blk = psx::xcd_logical_block(blockIdx.x, gridDim.x);
+  // a.rs_tile_major (resampling first pass, gridDim.y == 1): the trials of
+  // one tile on consecutive hardware blocks, so that the trials of a batch
+  // resample the same stretch of their DM's series at about the same time
+  // and the re-reads hit the caches (trial-major order reads a series too
+  // large for them, such as a 2^26-sample one, from HBM again for every
+  // trial; for smaller series trial-major order is the faster one).
+  unsigned blk, tr;
+  if (RS && a.rs_tile_major) {
+    const unsigned hw = blockIdx.x + gridDim.x * blockIdx.z;
+    tr = hw % gridDim.z;
+    blk = hw / gridDim.z;
+  } else {
+    blk = psx::xcd_logical_block(blockIdx.x, gridDim.x);
+    tr = blockIdx.z;
+  }
   const unsigned col0 = blk * C;
-  float2* out = a.out + (int64_t)blockIdx.y * a.out_sub + (int64_t)blockIdx.z * a.out_tr;
+  float2* out = a.out + (int64_t)blockIdx.y * a.out_sub + (int64_t)tr * a.out_tr;
   const int t = threadIdx.x;
   const uint32_t icol = (uint32_t)a.in_col, iel = (uint32_t)a.in_elem;
   float2 v[P];
@@ -241,7 +255,7 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
   if constexpr (RS) {
     // resampleII fused into the first read (src/kernels.cu:342-353); the
     // complex input m is the sample pair (2m, 2m+1) of the resampled series
-    const TrialDesc d = a.descs[blockIdx.z];
+    const TrialDesc d = a.descs[tr];
     const float* __restrict__ tim = d.tim;
     const double af = d.af;
     const uint32_t n = (uint32_t)a.rs_n;
@@ -270,7 +284,7 @@ __attribute__((amdgpu_waves_per_eu(waves_per_simd(L, C, MODE)))) k_fft_pass(cons
   } else {
     // every input element is read once: streaming loads (0.4% end to end
     // at 2^23 on top of the streaming row stores)
-    const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)blockIdx.z * a.in_tr;
+    const float2* in = a.in + (int64_t)blockIdx.y * a.in_sub + (int64_t)tr * a.in_tr;
     auto load = [&](unsigned cbase, float2(&dst)[P]) {
 #pragma unroll
       for (int b = 0; b < NB0; ++b) {
@@ -379,6 +393,11 @@ struct KernelInfo {
 // 4,096-point row pass at 2^23, 64 trials: 1 tile 0.97 ms, 2 tiles 0.85,
 // 4 tiles 0.85, 8 tiles 0.82 per launch).
 constexpr int kTilesPerGroup = 8;
+// Series size from which the resampling first pass runs tile-major
+// (same-box end-to-end rates, trial-major against tile-major: 2^23 27,280
+// against 25,960 trials/s, 2^24 12,390 against 11,590, 2^25 2,986 against
+// 2,998, 2^26 2,266 against 2,311).
+constexpr uint64_t kTileMajorSeriesBytes = 256ull << 20;
 
 template <int L, int C, int MODE, bool RS>
 constexpr KernelInfo kinfo1() {
@@ -488,25 +507,56 @@ bool default_decomp(uint64_t M, Decomp& d) {
     d.C[0] = 1;
     return true;
   }
-  uint64_t best = 0, bM1 = 0, bM2 = 0;
+  uint64_t bM1 = 0, bM2 = 0;
   for (uint64_t M1 = 32; M1 <= 4096 && M1 < M; M1 = (M1 % 3 == 0) ? (M1 / 3) * 4 : M1 + M1 / 2) {
     if (M % M1) continue;
     const uint64_t M2 = M / M1;
     if (!len_supported(M1, true) || !len_supported(M2, true)) continue;
     if (M1 > bM1) {
-      best = M1;
       bM1 = M1;
       bM2 = M2;
     }
   }
-  if (best) {
+  auto two = [&]() {
     d.npass = 2;
     d.L[0] = (int)bM2;
     d.L[1] = (int)bM1;
     return pick_tiles(d);
+  };
+  // two passes while the strided first pass is at most 2,048 points (8+
+  // column tiles, 64-byte runs); a 4,096-point strided pass runs 4-column
+  // tiles, 32-byte runs, and three passes with a long contiguous last pass
+  // are faster (2^25 samples: 4,096 x 4,096 3,000 trials/s, 64 x 64 x 4,096
+  // 5,090, same box)
+  if (bM1 && bM2 <= 2048) return two();
+  // three passes: the longest contiguous last pass (row-layout kernels up to
+  // 4,096 points) and two short strided passes of at most 512 points, as
+  // even as possible with the first (resampling) pass the longer one
+  // (2^26 samples: 128 x 64 x 4,096 2,517 trials/s, 512 x 128 x 512 2,343)
+  for (uint64_t M1 : {4096ull, 3072ull, 2048ull, 1536ull, 1024ull}) {
+    if (M % M1 || !len_supported(M1, true)) continue;
+    const uint64_t rest = M / M1;
+    uint64_t bq = 0, bm = 0;
+    for (uint64_t M2 = 32; M2 <= 512; M2 = (M2 % 3 == 0) ? (M2 / 3) * 4 : M2 + M2 / 2) {
+      if (rest % M2 || !len_supported(M2, true)) continue;
+      const uint64_t Q = rest / M2;
+      if (Q < M2 || Q > 512 || !len_supported(Q, true)) continue;
+      if (!bq || Q < bq) {
+        bq = Q;
+        bm = M2;
+      }
+    }
+    if (bq) {
+      d.npass = 3;
+      d.L[0] = (int)bq;
+      d.L[1] = (int)bm;
+      d.L[2] = (int)M1;
+      return pick_tiles(d);
+    }
   }
-  uint64_t bQ = 0;
-  best = 0;
+  if (bM1) return two();
+  // otherwise the three passes whose longest is shortest
+  uint64_t best = 0, bQ = 0;
   for (uint64_t M1 = 32; M1 <= 4096; M1 = (M1 % 3 == 0) ? (M1 / 3) * 4 : M1 + M1 / 2) {
     if (M % M1 || !len_supported(M1, true)) continue;
     for (uint64_t M2 = 32; M2 <= 4096; M2 = (M2 % 3 == 0) ? (M2 / 3) * 4 : M2 + M2 / 2) {
@@ -871,12 +921,15 @@ int exec_fused_pass(psx_fft_plan* p, int q, const TrialDesc* d_descs, uint32_t n
     a.resample = 1;
     a.rs_n = p->n;
     a.descs = d_descs;
+    a.rs_tile_major = p->n * sizeof(float) >= kTileMajorSeriesBytes ? 1 : 0;
   }
   const uint64_t cap_in = p->M * p->batch, cap_out = L.dst == 2 ? p->M * p->batch : p->M * p->batch;
   int rc = check_pass_bounds(L, a, ntrials, q == 0 ? 0 : cap_in, cap_out, q == 0 ? p->M : 0);
   if (rc) return rc;
   const KernelInfo* k = q == 0 ? L.krs : L.k;
   if (!k) return psx::fail(PSX_ERR_UNSUPPORTED, "exec_fused_pass: no resampling kernel for this pass");
+  // the resampling kernel orders its blocks tile-major over x and z only
+  if (q == 0 && L.gy != 1) return psx::fail(PSX_ERR_UNSUPPORTED, "exec_fused_pass: first pass with sub-transforms");
   unsigned gx = L.gx;
   if (k->two_tiles && gx > 1) {
     a.tiles = kTilesPerGroup;
@@ -1007,6 +1060,14 @@ extern "C" int psx_fft_layout_check(uint64_t n, int kind, uint32_t batch, int np
       rc = psx::fft::check_pass_bounds(L, a, p.batch, q == 0 ? 0 : cap_in, p.M * p.batch, q == 0 ? p.M : 0);
       if (rc) return rc;
     }
+  }
+  if (kind == PSX_FFT_R2C && p.has_rows_last) {
+    // the fused search's row-layout last pass (half-spectra back to back)
+    PassArgs a = p.rows_last.a;
+    a.out_tr = (int64_t)p.M;
+    const uint64_t cap = p.M * p.batch;
+    rc = psx::fft::check_pass_bounds(p.rows_last, a, p.batch, p.passes.size() == 1 ? 0 : cap, cap, 0);
+    if (rc) return rc;
   }
   return psx::ok();
 }

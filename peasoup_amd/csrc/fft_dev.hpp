@@ -300,6 +300,7 @@ struct PassArgs {
   // trial blockIdx.z is (tim[idx(2m)], tim[idx(2m+1)]) with the resample index
   // of the trial's af over a series of rs_n samples.
   int32_t resample;
+  int32_t rs_tile_major;  // block order of the resampling pass (see k_fft_pass)
   uint64_t rs_n;
   const TrialDesc* descs;
   // Several column tiles per workgroup (kernels where two_tiles_ok): the

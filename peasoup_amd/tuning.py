@@ -72,7 +72,8 @@ LENS = sorted(TILES)
 
 def candidates(n, limit=12):
     """Two-pass decompositions (M2 strided, M1 contiguous) of n/2 (2^k or
-    3*2^k), then three-pass ones of short 16-column passes."""
+    3*2^k), then three-pass ones: short 16-column passes, and two short
+    strided passes before a long contiguous one."""
     m = n // 2
     out = []
     for m1 in LENS:
@@ -92,6 +93,14 @@ def candidates(n, limit=12):
             m1 = m // (q * m2)
             if q * m2 * m1 == m and m1 in short:
                 out.append([(q, 16), (m2, 16), (m1, 16)])
+    # two short strided passes and a long contiguous last pass (the shape of
+    # the default plan beyond 2^24 samples)
+    for m1 in (4096, 3072, 2048, 1536, 1024):
+        for q in short:
+            for m2 in short:
+                if q * m2 * m1 == m and q >= m2:
+                    for c1 in TILES[m1][:2]:
+                        out.append([(q, 16), (m2, 16), (m1, c1)])
     return out
 
 
