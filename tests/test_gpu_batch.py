@@ -346,10 +346,12 @@ def test_batched_overflow_is_an_error(psx):
         W.batch_search(args, series, w.dm_list(), None, W.make_batch_opts(1, cand_capacity=10000))
 
 
-def test_batched_2pow26_three_pass_matches_worker(psx):
-    """Long-segment length 2^26 (three FFT passes, fused first pass)."""
+@pytest.mark.parametrize("n", [1 << 25, 1 << 26])
+def test_batched_long_three_pass_matches_worker(psx, n):
+    """Lengths 2^25 and 2^26 (three FFT passes, fused first pass, row-layout
+    hand-over; tile-major resampling at 2^26) against the plain Worker."""
     import peasoup_amd.workloads as W
-    w = W.Workload("t", 1, 1 << 26, 64e-6, 41, 5.0, -50, 50, injections=[W.Injection(0, 0.0023, 12.0, 25.0, 0.7)])
+    w = W.Workload("t", 1, n, 64e-6, 41, 5.0, -50, 50, injections=[W.Injection(0, 0.0023, 12.0, 25.0, 0.7)])
     series = W.generate_host(w)
     full = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
     k = int(np.argmin(np.abs(full - 12.0)))  # the trials around the injected acceleration
@@ -360,10 +362,13 @@ def test_batched_2pow26_three_pass_matches_worker(psx):
     assert np.any(got["snr"] > 10)
 
 
-def test_fft_2pow26_vs_exact(psx, orc):
-    n = 1 << 26
-    x = np.random.default_rng(26).standard_normal(n).astype(np.float32)
+@pytest.mark.parametrize("n,plan_desc", [(1 << 25, "64/C16x64/C16x4096/C4"), (1 << 26, "128/C16x64/C16x4096/C4")])
+def test_fft_long_default_plans_vs_exact(psx, orc, n, plan_desc):
+    """The default plans beyond 2^24 samples (two short strided passes and
+    a 4,096-point contiguous last pass) against an exact double DFT."""
+    x = np.random.default_rng(n.bit_length()).standard_normal(n).astype(np.float32)
     plan = psx.FFTPlan(n, psx.PSX_FFT_R2C)
+    assert plan_desc in plan.describe(), plan.describe()
     d_x, d_X = psx.DeviceArray.from_numpy(x), psx.DeviceArray(n // 2 + 1, np.complex64)
     plan.execute(d_x, d_X)
     got = d_X.numpy()
