@@ -634,9 +634,10 @@ struct psx_fft_plan {
   uint32_t batch = 1;
   psx::fft::Decomp d;
   std::vector<PassLaunch> passes;
-  // Two-pass R2C plans whose last pass writes narrow column runs (C < 16)
-  // also carry a version of that pass writing contiguous rows: Y[r*M1 + e]
-  // = Z[r + M2*e], the transposed layout the batched spectrum pass reads.
+  // Multi-pass R2C plans whose last pass has a contiguous-row kernel also
+  // carry a version of that pass writing contiguous rows: Y[r*L + e] =
+  // Z[r + R*e] (L the last pass's length, R = M/L), the transposed layout
+  // the batched spectrum pass reads.
   PassLaunch rows_last{};
   bool has_rows_last = false;
   float2* tables = nullptr;  // all device tables in one allocation
