@@ -12,7 +12,7 @@ import peasoup_amd as P
 from peasoup_amd import tuning
 
 SIZES = [16, 32, 64, 96, 1024, 6144, 1 << 13, 1 << 14, 1 << 16, 1 << 17, 3 << 17, 1 << 20, 1 << 22,
-         3 << 21, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28]
+         3 << 21, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28, 3 << 23, 3 << 24, 3 << 25]
 
 
 def check(n, kind, batch, dec=None):
