@@ -269,8 +269,8 @@ typedef struct psx_batch_opts {
   int32_t harmonic_pass;   /* PSX_HARMONIC_*: form of the harmonic-sum pass */
 } psx_batch_opts;
 
-/* Spectrum hand-over of the batched path.  AUTO: a two-pass plan whose last
- * pass has column tiles narrower than 16 writes contiguous rows and the
+/* Spectrum hand-over of the batched path.  AUTO: a multi-pass plan whose
+ * last pass has a contiguous-row kernel writes contiguous rows and the
  * spectrum pass reads that transposed layout; NATURAL: always natural bin
  * order; ROWS: the transposed layout, an error when the plan has none.  All
  * three give identical spectra for the same FFT decomposition. */

@@ -496,8 +496,8 @@ static bool len_supported(uint64_t L, bool multi) {
 // Default decomposition: one pass when a single-workgroup kernel exists,
 // else the two-pass split with the longest contiguous (last) pass, whose
 // rows are read and written whole, and the shortest strided pass (measured
-// fastest on MI355X: 1024 x 4096 beats 2048 x 2048 for 2^22 points), else the
-// most balanced three-pass split.
+// fastest on MI355X: 1024 x 4096 beats 2048 x 2048 for 2^22 points) while
+// that strided pass is at most 2,048 points, else three passes as below.
 bool default_decomp(uint64_t M, Decomp& d) {
   d = Decomp();
   if (!valid_complex_len(M)) return false;
