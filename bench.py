@@ -267,8 +267,11 @@ def long_segment(P, W, ndm=2, batch=16, log=log):
     c, s = W.batch_search(w.search_args(), d, w.dm_list(), accs, opts, nsamps=w.size)
     dt = time.perf_counter() - t0
     P.lib().psx_batch_release()
+    # per-launch stage times of one full batch (HIP events), after the timing
+    ms, desc = profile_stages(P, W, w.search_args(), W.make_batch_opts(batch), d.ptr, w.size, accs[:batch], reps=2)
     n = ndm * accs.size
     res = {"trials": n, "seconds": round(dt, 3), "trials_per_s": round(n / dt, 1), "candidates": int(c.size),
+           "fft_plan": desc, "stage_ms_per_launch": [round(x, 3) for x in ms],
            "sample": "%d of the 32 DM trials x 2^26 samples (67,108,864), %d accelerations each, batch %d, "
                      "engine kept from the warm-up batch" % (ndm, accs.size, batch)}
     log("long segment", res)
