@@ -257,7 +257,8 @@ int psx_worker_search(const psx_search_args* args, const float* series, uint64_t
 /* Options of the batched search (psx_batch_search).  Zero fields take
  * defaults; fft_npass > 0 forces an FFT decomposition (autotuner). */
 typedef struct psx_batch_opts {
-  uint32_t batch;          /* trials per fused batch (default 16) */
+  uint32_t batch;          /* trials per fused batch (0: 64 up to 2^26 samples,
+                              32 at 2^27, 16 at 2^28) */
   uint32_t cand_capacity;  /* above-threshold bins per batch (default 2^20) */
   int32_t use_graph;       /* launch each batch as one captured hipGraph */
   int32_t fft_npass;       /* 0 = default decomposition */
@@ -345,7 +346,8 @@ typedef struct psx_schedule {
 
 typedef struct psx_server_config {
   uint64_t hbm_budget_bytes;  /* resident series + engine buffers never exceed this */
-  uint32_t batch;             /* trials per fused batch (default 16) */
+  uint32_t batch;             /* trials per fused batch (0: by length, as in
+                                 psx_batch_opts) */
   uint32_t max_active;        /* admitted requests at once (0 = budget only) */
   int32_t use_graph;          /* graph-launched batches */
   uint32_t nschedules;        /* per-length schedules (copied at create; may be 0) */

@@ -98,7 +98,8 @@ void run_trials(Engine& eng, const psx_search_args& a, const std::vector<const f
 }  // namespace psx
 
 namespace {
-bool engine_options(const psx_batch_opts* opts, EngineOptions& eo, psx::fft::Decomp& forced) {
+bool engine_options(const psx_batch_opts* opts, uint64_t n, EngineOptions& eo, psx::fft::Decomp& forced) {
+  eo.batch = psx::batch::default_batch(n);
   if (opts) {
     eo.batch = opts->batch ? opts->batch : eo.batch;
     eo.cand_capacity = opts->cand_capacity ? opts->cand_capacity : eo.cand_capacity;
@@ -215,7 +216,7 @@ extern "C" int psx_batch_profile(const psx_search_args* args, const psx_batch_op
   try {
     EngineOptions eo;
     psx::fft::Decomp forced;
-    engine_options(opts, eo, forced);
+    engine_options(opts, args->size, eo, forced);
     PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
     PSX_CHECK_HIP(hipMalloc(&d_tim, sizeof(float) * args->size));
@@ -256,7 +257,7 @@ extern "C" int psx_batch_bound_audit(const psx_search_args* args, const psx_batc
   try {
     EngineOptions eo;
     psx::fft::Decomp forced;
-    engine_options(opts, eo, forced);
+    engine_options(opts, args->size, eo, forced);
     PSX_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     std::unique_ptr<Engine> eng(new Engine(*args, eo, s));
     PSX_CHECK_HIP(hipMalloc(&d_tim, sizeof(float) * args->size));
@@ -311,7 +312,7 @@ extern "C" int psx_batch_search(const psx_search_args* args, const psx_batch_opt
   try {
     EngineOptions eo;
     psx::fft::Decomp forced;
-    engine_options(opts, eo, forced);
+    engine_options(opts, args->size, eo, forced);
     const bool keep_engine = opts && opts->keep_engine;
     const EngineKey key = engine_key(*args, eo);
     const uint64_t N = args->size;

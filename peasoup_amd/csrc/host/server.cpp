@@ -135,7 +135,8 @@ struct psx_server {
   bool admit_one(Request* r);
   uint32_t batch_for(uint64_t size) const {
     auto it = schedules.find(size);
-    return (it != schedules.end() && it->second.batch) ? it->second.batch : cfg.batch;
+    if (it != schedules.end() && it->second.batch) return it->second.batch;
+    return cfg.batch ? cfg.batch : psx::batch::default_batch(size);
   }
   void finish(Request* r);
   void fail_request(Request* r, int code, const std::string& msg);
@@ -463,7 +464,6 @@ extern "C" int psx_server_create(const psx_server_config* cfg, psx_server** out)
               "psx_server_create: dispatch must be PSX_DISPATCH_STREAMS, or PSX_DISPATCH_MIXED_GRAPH with graphs");
   psx_server* s = new psx_server;
   s->cfg = *cfg;
-  if (!s->cfg.batch) s->cfg.batch = 16;
   for (uint32_t i = 0; i < cfg->nschedules; ++i) {
     const psx_schedule& sc = cfg->schedules[i];
     if (sc.batch > 1024 || sc.fft_npass < 0 || sc.fft_npass > 3) {

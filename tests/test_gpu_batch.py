@@ -23,7 +23,8 @@ def cfg1(psx):
     return w, W.generate_host(w), W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
 
 
-@pytest.mark.parametrize("batch,graph", [(16, True), (3, True), (1, False), (5, False)])
+# batch 0: the library's default for the length (64 trials per launch here)
+@pytest.mark.parametrize("batch,graph", [(16, True), (3, True), (1, False), (5, False), (0, True)])
 def test_batched_equals_worker(psx, cfg1, batch, graph):
     import peasoup_amd.workloads as W
     w, series, accs = cfg1
