@@ -96,12 +96,14 @@ def test_parity_uneven_segments(psx, orc, n):
     _check(psx, orc, w, accs, 1 / 0.0061)
 
 
-def test_parity_2pow26_long_segment(psx, orc):
-    """Long-segment workload: 2^26 samples at 64 us with red noise, one binary
-    pulsar; 1 DM trial x 3 accelerations around it (batch 16, the tuned
-    three-pass plan)."""
+@pytest.mark.parametrize("n", [1 << 25, 1 << 26])
+def test_parity_long_segments(psx, orc, n):
+    """Long segments: 2^25 and 2^26 samples at 64 us with red noise, one
+    binary pulsar; 1 DM trial x 3 accelerations around it (the default
+    three-pass plans: two short strided passes and a 4,096-point contiguous
+    one; tile-major resampling at 2^26)."""
     import peasoup_amd.workloads as W
-    w = W.Workload("long", 1, 1 << 26, 64e-6, 0x5EA5C4, 5.0, -50.0, 50.0)
+    w = W.Workload("long", 1, n, 64e-6, 0x5EA5C4, 5.0, -50.0, 50.0)
     accs_all = W.accel_list(w.size, w.tsamp, w.acc_lo, w.acc_hi)
     k = int(np.argmin(np.abs(accs_all - 21.0)))
     w.injections = [W.Injection(0, 0.00427, float(accs_all[k]), 25.0, 0.1)]
