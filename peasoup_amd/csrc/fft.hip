@@ -418,7 +418,7 @@ const KernelInfo kKernels[] = {
     PSX_K1(2048), PSX_K1(4096), PSX_K1(6), PSX_K1(12), PSX_K1(24), PSX_K1(48), PSX_K1(96), PSX_K1(192),
     PSX_K1(384), PSX_K1(768), PSX_K1(1536), PSX_K1(3072),
     PSX_KC(32, 16), PSX_KC(64, 16), PSX_KC(128, 16), PSX_KC(256, 16), PSX_KC(512, 16), PSX_KC(1024, 16),
-    PSX_KC(64, 32), PSX_KC(128, 32),
+    PSX_KC(64, 128), PSX_KC(128, 128), PSX_KC(64, 64), PSX_KC(128, 64), PSX_KC(64, 32), PSX_KC(128, 32),
     PSX_KC(64, 8), PSX_KC(128, 8), PSX_KC(256, 8), PSX_KC(512, 8), PSX_KC(1024, 8), PSX_KC(2048, 8),
     PSX_KC(256, 4), PSX_KC(512, 4), PSX_KC(1024, 4), PSX_KC(2048, 4), PSX_KC(4096, 4),
     PSX_KC(1024, 2), PSX_KC(2048, 2), PSX_KC(4096, 2),
@@ -585,17 +585,18 @@ bool default_decomp(uint64_t M, Decomp& d) {
 
 // Widest column tile whose LDS fits 80 KiB (two workgroups per CU) for the
 // strided passes; the last pass, which reads and writes whole rows, may use
-// a CU's whole LDS.  The short strided passes of three-pass plans take
-// 32-column tiles where those kernels exist (64 and 128 points: 256-byte
-// runs; 2^26 samples 2,606 against 2,569 trials/s with 16 columns, 2^25
-// 5,208 against 5,101, same box, same candidates).
+// a CU's whole LDS.  The short strided passes of three-pass plans take the
+// widest tiles that exist for them (64 and 128 points: 128 columns, 1 KB
+// runs; same-box end to end at 2^26 with 16 / 32 / 64 / 128 columns 2,569 /
+// 2,606 / 2,695 / 2,715 trials/s, at 2^25 5,101 / 5,208 / 5,250 / 5,318,
+// same candidates).
 bool pick_tiles(Decomp& d) {
   for (int p = 0; p < d.npass; ++p) {
     if (d.C[p] > 0 && find_kernel(d.L[p], d.C[p])) continue;
-    const int pref3[] = {32, 16, 8, 4, 2}, pref2[] = {16, 8, 4, 2};
+    const int pref3[] = {128, 64, 32, 16, 8, 4, 2}, pref2[] = {16, 8, 4, 2};
     const bool last = (p == d.npass - 1 && d.npass > 1);
     const int* pref = (d.npass == 3 && !last) ? pref3 : pref2;
-    const int npref = (d.npass == 3 && !last) ? 5 : 4;
+    const int npref = (d.npass == 3 && !last) ? 7 : 4;
     const uint32_t cap = last ? 160u * 1024u : 80u * 1024u;
     const int mode = last ? PM_ROWS_TO_COLS : PM_COLS_TW;
     d.C[p] = 0;

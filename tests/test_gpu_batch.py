@@ -363,7 +363,7 @@ def test_batched_long_three_pass_matches_worker(psx, n):
     assert np.any(got["snr"] > 10)
 
 
-@pytest.mark.parametrize("n,plan_desc", [(1 << 25, "64/C32x64/C32x4096/C4"), (1 << 26, "128/C32x64/C32x4096/C4")])
+@pytest.mark.parametrize("n,plan_desc", [(1 << 25, "64/C128x64/C128x4096/C4"), (1 << 26, "128/C128x64/C128x4096/C4")])
 def test_fft_long_default_plans_vs_exact(psx, orc, n, plan_desc):
     """The default plans beyond 2^24 samples (two short strided passes and
     a 4,096-point contiguous last pass) against an exact double DFT."""
