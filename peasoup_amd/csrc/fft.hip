@@ -395,8 +395,8 @@ struct KernelInfo {
 constexpr int kTilesPerGroup = 8;
 // Series size from which the resampling first pass runs tile-major
 // (same-box end-to-end rates, trial-major against tile-major: 2^23 27,280
-// against 25,960 trials/s, 2^24 12,390 against 11,590, 2^25 2,986 against
-// 2,998, 2^26 2,266 against 2,311).
+// against 25,960 trials/s, 2^24 12,390 against 11,590; with the default
+// long plans 2^25 5,427 against 5,323 and 2^26 2,568 against 2,738).
 constexpr uint64_t kTileMajorSeriesBytes = 256ull << 20;
 
 template <int L, int C, int MODE, bool RS>
